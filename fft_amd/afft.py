@@ -1,0 +1,191 @@
+"""Python mirror of afft's C++ wrapper classes (fft.hpp:47-234) over the GPU
+C ABI: ``complex_transform``, ``inverse_complex_transform``, ``real_transform``
+and ``inverse_real_transform``, each templated in afft on the element type and
+the format.  Here the element type is ``dtype`` (numpy float32 / float64) and
+the buffers are device buffers (torch tensors on cuda or raw device addresses).
+
+As in afft, a failed create leaves the object falsy (fft.hpp:68) instead of
+raising; calling a falsy transform raises.  ``close()`` releases the device
+resources (the reference has nothing to release).  ``Plan`` is the batched,
+stream-aware extension (include/afft_gpu.h)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import afft_auto, afft_interleaved, afft_split  # noqa: F401
+
+
+def _prefix(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "32"
+    if dt == np.float64:
+        return "64"
+    raise TypeError(f"unsupported element type {dt}")
+
+
+class _Transform:
+    _kind = "c"
+
+    def __init__(self, dims, dtype=np.float32, format=afft_split, impl=afft_auto):
+        self.dims = [int(d) for d in dims]
+        self.format = int(format)
+        self.dtype = np.dtype(dtype)
+        p = _prefix(dtype)
+        L = _lib.lib()
+        nd, arr = _lib.dims_array(self.dims)
+        self._fn_transform = getattr(L, f"afft_{p}_{self._kind}_transform")
+        self._fn_destroy = getattr(L, f"afft_{p}_{self._kind}_destroy")
+        size = getattr(L, f"afft_{p}_{self._kind}_memsize")(nd, arr, self.format, int(impl))
+        self.memsize = size
+        self._mem = None
+        self.state = None
+        if size == 0 or size == ctypes.c_size_t(-1).value:
+            return
+        self._mem = ctypes.create_string_buffer(size)
+        st = getattr(L, f"afft_{p}_{self._kind}_create")(nd, arr, self.format, self._mem, int(impl))
+        self.state = st if st else None
+
+    def __bool__(self):
+        return self.state is not None
+
+    def _check(self):
+        if not self.state:
+            raise RuntimeError(f"invalid transform: {_lib.last_error()}")
+
+    def close(self):
+        if self.state:
+            self._fn_destroy(self.state)
+        self.state = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream) -> None:
+        """Run on a HIP stream (torch.cuda.Stream or raw handle)."""
+        self._check()
+        h = getattr(stream, "cuda_stream", stream)
+        if _lib.lib().afft_gpu_set_stream(self.state, h) != 0:
+            raise RuntimeError(_lib.last_error())
+
+
+class complex_transform(_Transform):
+    """afft::complex_transform (fft.hpp:47-118): forward c2c."""
+    _kind = "c"
+
+    def __call__(self, src_re, src_im, dst_re, dst_im):
+        self._check()
+        self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re), _lib.ptr(dst_im))
+
+
+class inverse_complex_transform(_Transform):
+    """afft::inverse_complex_transform (fft.hpp:120-164): unnormalised inverse c2c."""
+    _kind = "ci"
+
+    def __call__(self, src_re, src_im, dst_re, dst_im):
+        self._check()
+        self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re), _lib.ptr(dst_im))
+
+
+class real_transform(_Transform):
+    """afft::real_transform (fft.hpp:166-200): r2c, dim 0 halved."""
+    _kind = "r"
+
+    def __call__(self, src, dst_re, dst_im):
+        self._check()
+        self._fn_transform(self.state, _lib.ptr(src), _lib.ptr(dst_re), _lib.ptr(dst_im))
+
+
+class inverse_real_transform(_Transform):
+    """afft::inverse_real_transform (fft.hpp:202-234): unnormalised c2r."""
+    _kind = "ri"
+
+    def __call__(self, src_re, src_im, dst):
+        self._check()
+        self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst))
+
+
+def supported(impl: int) -> bool:
+    return bool(_lib.lib().afft_supported(int(impl)))
+
+
+def align_size(n: int, dtype=np.float32) -> int:
+    p = _prefix(dtype)
+    return int(getattr(_lib.lib(), f"afft_{p}_align_size")(int(n)))
+
+
+class Plan:
+    """Batched asynchronous plan (afft_gpu_plan_create / afft_gpu_execute)."""
+
+    def __init__(self, kind: str, dims, batch: int = 1, dtype=np.float32, format=afft_interleaved):
+        L = _lib.lib()
+        self.kind = kind
+        self.dims = [int(d) for d in dims]
+        self.batch = int(batch)
+        self.format = int(format)
+        prec = 32 if np.dtype(dtype) == np.float32 else 64
+        nd, arr = _lib.dims_array(self.dims)
+        self._mem = ctypes.create_string_buffer(L.afft_gpu_plan_memsize())
+        st = L.afft_gpu_plan_create(prec, _lib.KINDS[kind], nd, arr, self.format, self.batch, self._mem)
+        if not st:
+            raise RuntimeError(f"plan creation failed: {_lib.last_error()}")
+        self.state = st
+
+    def __call__(self, src_re, src_im, dst_re, dst_im, stream=None):
+        h = getattr(stream, "cuda_stream", stream)
+        r = _lib.lib().afft_gpu_execute(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re),
+                                        _lib.ptr(dst_im), h)
+        if r != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def run_range(self, first, last, src_re, src_im, dst_re, dst_im, stream=None):
+        """Enqueues passes [first, last) only (for per-kernel timing)."""
+        h = getattr(stream, "cuda_stream", stream)
+        r = _lib.lib().afft_gpu_execute_range(self.state, int(first), int(last), _lib.ptr(src_re), _lib.ptr(src_im),
+                                              _lib.ptr(dst_re), _lib.ptr(dst_im), h)
+        if r != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def launches(self, pass_index: int) -> int:
+        return _lib.lib().afft_gpu_plan_launches(self.state, int(pass_index))
+
+    def set_chunk(self, chunk: int):
+        if _lib.lib().afft_gpu_plan_set_chunk(self.state, int(chunk)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def set_mode(self, mode: int):
+        if _lib.lib().afft_gpu_plan_set_mode(self.state, int(mode)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    @property
+    def num_passes(self) -> int:
+        return _lib.lib().afft_gpu_plan_num_passes(self.state)
+
+    @property
+    def scratch_bytes(self) -> int:
+        return _lib.lib().afft_gpu_plan_scratch_bytes(self.state)
+
+    def describe(self):
+        out = []
+        buf = ctypes.create_string_buffer(512)
+        for i in range(self.num_passes):
+            _lib.lib().afft_gpu_plan_describe(self.state, i, buf, 512)
+            out.append(buf.value.decode())
+        return out
+
+    def close(self):
+        if self.state:
+            _lib.lib().afft_gpu_plan_destroy(self.state)
+        self.state = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,46 @@
+// afft_aux.h -- declarations of the auxiliary kernels in afft_aux.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "afft_device.h"
+
+namespace afgpu {
+
+struct StageList {
+  int nstage;
+  int total;
+  int radix[8];
+  int ns[8];
+  int off[8];
+};
+
+template <typename T>
+struct RealRowsArgs {
+  const T* in_re;
+  const T* in_im;
+  T* out_re;
+  T* out_im;
+  int in_il, out_il;
+  idx_t in_rs, in_cs, in_bs;    // row, column, batch strides (elements)
+  idx_t out_rs, out_cs, out_bs;
+  idx_t h, m, nbatch;
+  const cx<T>* real_tw;  // W_{2h}^k, k < h
+};
+
+template <typename T>
+hipError_t gen_stage_tw(cx<T>* out, const StageList& sl, hipStream_t st);
+template <typename T>
+hipError_t gen_tw4(cx<T>* hi, cx<T>* lo, idx_t N, int bits, hipStream_t st);
+template <typename T>
+hipError_t gen_real_tw(cx<T>* out, idx_t L, hipStream_t st);
+template <typename T>
+hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st);
+template <typename T>
+hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
+                    hipStream_t st);
+template <typename TY>
+hipError_t err_sums(const TY* y, const double* r, idx_t n, double* acc, hipStream_t st);
+hipError_t copy16(const void* src, void* dst, idx_t bytes, hipStream_t st);
+
+}  // namespace afgpu

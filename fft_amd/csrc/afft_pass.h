@@ -1,0 +1,278 @@
+// afft_pass.h -- the batched LDS Stockham pass kernel.
+//
+// One work-group computes CT independent L-point FFTs ("columns").  Element j of
+// column c of batch entry b is read from
+//     in + b*in_bs + c*in_cs + j*in_js            (complex-element strides)
+// and output bin k is written to
+//     out + b*out_bs + c*out_cs + k*out_ks.
+// This one addressing form covers every pass the plans need:
+//   - contiguous 1-D rows (the whole transform for L <= 16384, the last axis
+//     of an N-D transform, pass B of a four-step transform),
+//   - strided columns (outer axes of N-D transforms -- the job afft's
+//     multi.hpp row passes do -- and pass A of a four-step transform),
+//   - the real-transform packings (rows 2j / 2j+1 of dim 0 read as re / im,
+//     fft_internal.hpp:326-330; N reals read as N/2 interleaved complex,
+//     onedim.hpp:1175-1189).
+//
+// Thread mapping.  Each of the TPT = L/E threads of a column holds E elements
+// j = t + m*TPT (m < E).  COLMAP=false puts t fastest across lanes (coalesced
+// when in_js == 1); COLMAP=true puts c fastest (coalesced when in_cs == 1).
+//
+// Stages.  Stockham autosort: stage s of radix R with Ns = R0*...*R(s-1) reads
+// entries j + r*L/R, multiplies by W_{Ns*R}^{r*(j mod Ns)}, runs the radix-R
+// butterfly and writes entry (j/Ns)*Ns*R + (j mod Ns) + r*Ns.  The first stage
+// reads straight from global memory and the last leaves bin t + m*TPT in
+// register m, so only the L-1 interior exchanges go through LDS and the output
+// is in natural order with no bit-reversal pass (afft needs one,
+// onedim.hpp:397-447).
+//
+// Epilogues (EPI):
+//   EPI_PLAIN  store.
+//   EPI_TW4    multiply bin k1 of column n2 by W_N^(n2*k1) before the store
+//              (pass A of the four-step N = N1*N2 split).
+//   EPI_R2C    forward real post-pass fused after the last stage (pairs bin k
+//              with L-k through LDS; writes L+1 bins): onedim.hpp:1031-1108 and
+//              multi.hpp:343-414, inverse=false.
+//   EPI_C2R    inverse real pre-pass fused into the load (reads bins k and L-k,
+//              L+1 input bins): the inverse=true branch of the same functions.
+// STORE_T routes the final store through LDS so a row-mapped pass can write
+// with columns fastest (pass B of a four-step transform writes bins k2*N1+k1).
+#pragma once
+
+#include "afft_device.h"
+
+namespace afgpu {
+
+enum { EPI_PLAIN = 0, EPI_TW4 = 1, EPI_R2C = 2, EPI_C2R = 3 };
+
+template <typename T>
+struct PassParams {
+  const T* in_re;
+  const T* in_im;
+  T* out_re;
+  T* out_im;
+  idx_t in_js, in_cs, in_bs;
+  idx_t out_ks, out_cs, out_bs;
+  idx_t in_bs2, out_bs2;  // stride of the low batch index (two-level batch)
+  int n_blo;              // batch entries of the low index (1 = one-level)
+  int n_col;              // columns per batch entry
+  int col_tiles;          // ceil(n_col / CT)
+  int n_tiles;            // col_tiles * n_batch
+  int in_il, out_il;
+  int swap_in, swap_out;
+  const cx<T>* stage_tw;  // per-length stage twiddles
+  const cx<T>* tw4_hi;    // W_N^(i << lo_bits)
+  const cx<T>* tw4_lo;    // W_N^j - 1, j < 2^lo_bits
+  int tw4_lo_bits;
+  int tw4_col_shift;      // n2 = column >> shift
+  const cx<T>* real_tw;   // W_{2L}^k, k < L
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> ld_elem(const T* __restrict__ re, const T* __restrict__ im, int il, idx_t e) {
+  if (il) return reinterpret_cast<const cx<T>*>(re)[e];
+  return {re[e], im[e]};
+}
+template <typename T>
+__device__ __forceinline__ void st_elem(T* __restrict__ re, T* __restrict__ im, int il, idx_t e, cx<T> v) {
+  if (il) {
+    reinterpret_cast<cx<T>*>(re)[e] = v;
+  } else {
+    re[e] = v.x;
+    im[e] = v.y;
+  }
+}
+
+// W_N^m from the two-level table: hi[m >> b] * (1 + lo[m & mask]).
+template <typename T>
+__device__ __forceinline__ cx<T> tw4_lookup(const cx<T>* __restrict__ hi, const cx<T>* __restrict__ lo, int bits,
+                                            idx_t m) {
+  cx<T> h = hi[m >> bits];
+  cx<T> l = lo[m & ((idx_t(1) << bits) - 1)];
+  return {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
+}
+
+template <typename T, int L, int S, typename LdsAt>
+__device__ __forceinline__ void run_stages(cx<T> (&v)[Sched<L>::E], int t, LdsAt& lat,
+                                           const cx<T>* __restrict__ tw) {
+  constexpr int E = Sched<L>::E;
+  constexpr int TPT = L / E;
+  constexpr int R = Sched<L>::R[S];
+  constexpr int NS = StageNs<L, S>::value;
+  constexpr int NB = E / R;
+  constexpr bool LAST = (S == Sched<L>::NST - 1);
+  static_for<NB>([&](auto bc) {
+    constexpr int b = decltype(bc)::value;
+    cx<T> a[R];
+    static_for<R>([&](auto rc) { a[decltype(rc)::value] = v[b + decltype(rc)::value * NB]; });
+    if constexpr (NS > 1) {
+      const int j = t + b * TPT;
+      const int k = j & (NS - 1);
+      const cx<T>* tp = tw + StageTwOff<L, S>::value + k * (R - 1);
+      static_for<R - 1>([&](auto rc) {
+        constexpr int r = decltype(rc)::value + 1;
+        a[r] = cmul(a[r], tp[r - 1]);
+      });
+    }
+    Dft<T, R>::run(a);
+    static_for<R>([&](auto rc) { v[b + decltype(rc)::value * NB] = a[decltype(rc)::value]; });
+  });
+  if constexpr (!LAST) {
+    static_for<NB>([&](auto bc) {
+      constexpr int b = decltype(bc)::value;
+      const int j = t + b * TPT;
+      const int base = (j / NS) * (NS * R) + (j & (NS - 1));
+      static_for<R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        lat(base + r * NS) = v[b + r * NB];
+      });
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[m] = lat(t + m * TPT);
+    });
+    __syncthreads();
+    run_stages<T, L, S + 1>(v, t, lat, tw);
+  }
+}
+
+template <int L, int CT>
+struct PassShape {
+  static constexpr int E = Sched<L>::E;
+  static constexpr int TPT = L / E;
+  static constexpr int NT = CT * TPT;
+  static constexpr int LL = LdsLen<L>::value;
+  static constexpr size_t lds_elems = size_t(CT) * LL;
+};
+
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
+__global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
+  using PS = PassShape<L, CT>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  constexpr int LL = PS::LL;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+
+  const int tid = threadIdx.x;
+  const int c = COLMAP ? (tid % CT) : (tid / TPT);
+  const int t = COLMAP ? (tid / CT) : (tid % TPT);
+  const int tile = blockIdx.x;
+  const int bidx = tile / p.col_tiles;
+  const int col0 = (tile - bidx * p.col_tiles) * CT;
+  const int col = col0 + c;
+  const bool active = col < p.n_col;
+  const int bhi = bidx / p.n_blo;
+  const int blo = bidx - bhi * p.n_blo;
+
+  auto lat = [&](int i) -> cx<T>& {
+    if constexpr (COLMAP)
+      return lds[lds_pad(i) * CT + c];
+    else
+      return lds[c * LL + lds_pad(i)];
+  };
+
+  cx<T> v[E];
+  const idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + (idx_t)col * p.in_cs;
+  if (active) {
+    if constexpr (EPI == EPI_C2R) {
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int k = t + m * TPT;
+        cx<T> s0 = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)k * p.in_js);
+        cx<T> s1 = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js);
+        cx<T> z;
+        if (k == 0) {
+          z = {s0.x + s1.x, s0.x - s1.x};
+        } else {
+          cx<T> w = p.real_tw[k];
+          cx<T> s1c = conjg(s1);
+          cx<T> a = s0 + s1c;
+          cx<T> bb = cmulc(s1c - s0, w);
+          z = a + mul_neg_i(bb);
+        }
+        v[m] = p.swap_in ? swapri(z) : z;
+      });
+    } else {
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        cx<T> z = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js);
+        v[m] = p.swap_in ? swapri(z) : z;
+      });
+    }
+  } else {
+    static_for<E>([&](auto mc) { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
+  }
+
+  run_stages<T, L, 0>(v, t, lat, p.stage_tw);
+
+  const idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)col * p.out_cs;
+  if constexpr (EPI == EPI_R2C) {
+    static_for<E>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      lat(t + m * TPT) = v[m];
+    });
+    __syncthreads();
+    if (active) {
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int k = t + m * TPT;
+        cx<T> s0 = v[m];
+        cx<T> x;
+        if (k == 0) {
+          x = {s0.x + s0.y, T(0)};
+          cx<T> xn = {s0.x - s0.y, T(0)};
+          st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn);
+        } else {
+          cx<T> s1c = conjg(lat(L - k));
+          cx<T> w = p.real_tw[k];
+          const T h = T(0.5);
+          cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
+          cx<T> d = s0 - s1c;
+          cx<T> bw = cmul(d, w);
+          cx<T> bb = {bw.x * h, bw.y * h};
+          x = a + mul_neg_i(bb);
+        }
+        st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x);
+      });
+    }
+  } else {
+    if constexpr (EPI == EPI_TW4) {
+      const idx_t n2 = (idx_t)(col >> p.tw4_col_shift);
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const idx_t k1 = t + m * TPT;
+        v[m] = cmul(v[m], tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, n2 * k1));
+      });
+    }
+    if constexpr (STORE_T) {
+      static_assert(!COLMAP, "transposed store is for row-mapped passes");
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        lat(t + m * TPT) = v[m];
+      });
+      __syncthreads();
+      const int c2 = tid % CT;
+      const int t2 = tid / CT;
+      const int colb = col0 + c2;
+      if (colb < p.n_col) {
+        const idx_t ob2 = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
+        static_for<E>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          const int k = t2 + m * TPT;
+          cx<T> z = lds[c2 * LL + lds_pad(k)];
+          st_elem(p.out_re, p.out_im, p.out_il, ob2 + (idx_t)k * p.out_ks, p.swap_out ? swapri(z) : z);
+        });
+      }
+    } else if (active) {
+      static_for<E>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        cx<T> z = v[m];
+        st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z);
+      });
+    }
+  }
+}
+
+}  // namespace afgpu

@@ -1,0 +1,1162 @@
+// afft_plan.hip -- plan construction, execution and the C ABI of fft.h /
+// afft_gpu.h.
+//
+// A plan is a short list of pass launches over three kinds of buffer: the
+// caller's src, the caller's dst and device scratch owned by the plan.  How
+// each afft operation decomposes (reference structure in brackets):
+//   c2c, 1-D, N <= 16384       one row pass          [onedim small_transform]
+//   c2c, 1-D, N  > 16384       four-step N = N1*N2: column pass of N1 with the
+//                              W_N twiddle fused (src -> scratch), then a row
+//                              pass of N2 storing columns-fastest (scratch ->
+//                              dst)                  [onedim large_transform]
+//   c2c, N-D                   last axis as above, every outer axis as one
+//                              strided column pass in place on dst
+//                              [fft_internal.hpp:121-152 + multi.hpp]
+//   r2c / c2r, 1-D             N/2-point c2c with the real pass fused into
+//                              the last / first stage (separate real-rows
+//                              kernel when N/2 > 16384) [onedim.hpp:1110-1254]
+//   r2c / c2r, N-D             dim 0 halved: fused column r2c/c2r along dim 0
+//                              plus the inner c2c on each of the N0/2+1 rows
+//                              [fft_internal.hpp:200-493]
+// Inverse transforms swap re/im on load and store of the forward passes.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/afft_gpu.h"
+#include "../../include/fft.h"
+#include "afft_aux.h"
+#include "afft_registry.h"
+
+using namespace afgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define HIP_OK(expr)                                                     \
+  do {                                                                   \
+    hipError_t e_ = (expr);                                              \
+    if (e_ != hipSuccess) {                                              \
+      set_err("%s failed: %s", #expr, hipGetErrorString(e_));            \
+      return false;                                                      \
+    }                                                                    \
+  } while (0)
+
+constexpr uint32_t kMagic = 0xAFF7C0DEu;
+constexpr int kMaxDim = 64;
+constexpr int kMaxPass = 160;
+
+enum Role { R_SRC = 0, R_DST = 1, R_SCR = 2, R_SCR2 = 3 };
+enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2 };
+
+struct Side {
+  int role = R_SRC;
+  int il = 1;         // interleaved (complex-element units) vs split (scalar units)
+  int one_array = 0;  // split addressing with re and im both in the re array
+  idx_t re_off = 0, im_off = 0;
+  idx_t s_j = 0, s_c = 0, s_b = 0, s_b2 = 0;
+  idx_t s_u = 0;      // offset between consecutive user transforms (src/dst only)
+};
+
+struct PassRec {
+  int type = PT_FFT;
+  int L = 1, CT = 1, colmap = 0, epi = EPI_PLAIN, store_t = 0;
+  const void* variant = nullptr;
+  Side in, out;
+  idx_t n_col = 1;
+  int fold_user = 0;     // user transforms folded into the column index
+  idx_t b_per_user = 1;  // batch entries per user transform
+  int n_blo = 1;
+  int swap_in = 0, swap_out = 0;
+  const void* stage_tw = nullptr;
+  const void* tw4_hi = nullptr;
+  const void* tw4_lo = nullptr;
+  int tw4_bits = 0, tw4_shift = 0;
+  const void* real_tw = nullptr;
+  idx_t rr_h = 0, rr_m = 0, rr_rs = 0;
+  int rr_inv = 0;
+};
+
+struct Plan {
+  int prec = 32;
+  int kind = 0;
+  int format = 1;
+  int ndim = 0;
+  idx_t dims[kMaxDim];
+  idx_t batch = 1;
+  idx_t chunk = 1;
+  idx_t scr_per_user[2] = {0, 0};  // complex elements of scratch per user transform
+  void* scr[2] = {nullptr, nullptr};
+  size_t scr_bytes[2] = {0, 0};
+  int npass = 0;
+  PassRec pass[kMaxPass];
+  hipStream_t stream = nullptr;
+  int device = 0;
+};
+
+struct Header {
+  uint32_t magic;
+  uint32_t prec;
+  Plan* plan;
+};
+
+bool read_header(const void* state, Header& h) {
+  if (!state) {
+    set_err("null plan");
+    return false;
+  }
+  std::memcpy(&h, state, sizeof(h));
+  if (h.magic != kMagic || !h.plan) {
+    set_err("not a plan created by this library");
+    return false;
+  }
+  return true;
+}
+
+// ---- device twiddle cache ---------------------------------------------------
+struct TwCache {
+  std::mutex mu;
+  std::map<std::tuple<int, int, int, long long, int>, void*> tabs;  // (device, prec, type, n, bits)
+};
+TwCache& twcache() {
+  static TwCache* c = new TwCache;  // never destroyed: device memory lives for the process
+  return *c;
+}
+
+thread_local bool g_dry = false;  // host-only plan inspection: no device calls
+
+template <typename T>
+const void* get_table(int type, idx_t n, int bits) {
+  if (g_dry) return (const void*)(uintptr_t)(0x1000 + type);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto key = std::make_tuple(dev, (int)sizeof(T), type, (long long)n, bits);
+  TwCache& c = twcache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.tabs.find(key);
+  if (it != c.tabs.end()) return it->second;
+  void* p = nullptr;
+  hipError_t e = hipSuccess;
+  if (type == 0) {  // stage twiddles of an n-point pass
+    int radix[8], ns[8];
+    int nst = stage_list((int)n, radix, ns);
+    StageList sl{};
+    sl.nstage = nst;
+    int off = 0;
+    for (int s = 0; s < nst; s++) {
+      sl.radix[s] = radix[s];
+      sl.ns[s] = ns[s];
+      sl.off[s] = off;
+      if (s >= 1) off += ns[s] * (radix[s] - 1);
+    }
+    sl.total = off;
+    size_t bytes = sizeof(cx<T>) * (size_t)(off > 0 ? off : 1);
+    e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) e = gen_stage_tw<T>((cx<T>*)p, sl, nullptr);
+  } else if (type == 1) {  // four-step twiddles, hi then lo
+    idx_t nhi = n >> bits, nlo = idx_t(1) << bits;
+    e = hipMalloc(&p, sizeof(cx<T>) * (size_t)(nhi + nlo));
+    if (e == hipSuccess) e = gen_tw4<T>((cx<T>*)p, (cx<T>*)p + nhi, n, bits, nullptr);
+  } else {  // real-pass twiddles W_{2n}^k, k < n
+    e = hipMalloc(&p, sizeof(cx<T>) * (size_t)(n > 0 ? n : 1));
+    if (e == hipSuccess) e = gen_real_tw<T>((cx<T>*)p, n, nullptr);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    set_err("twiddle table: %s", hipGetErrorString(e));
+    if (p) (void)hipFree(p);
+    return nullptr;
+  }
+  c.tabs[key] = p;
+  return p;
+}
+
+int ilog2(idx_t v) {
+  int r = 0;
+  while ((idx_t(1) << r) < v) r++;
+  return r;
+}
+bool is_pow2(idx_t v) { return v >= 1 && (v & (v - 1)) == 0; }
+
+// ---- plan construction --------------------------------------------------------
+template <typename T>
+struct Builder {
+  Plan& P;
+  bool ok = true;
+  explicit Builder(Plan& p) : P(p) {}
+
+  PassRec* add() {
+    if (P.npass >= kMaxPass) {
+      set_err("too many passes");
+      ok = false;
+      return nullptr;
+    }
+    PassRec* r = &P.pass[P.npass++];
+    *r = PassRec();
+    return r;
+  }
+
+  void need_scratch(int role, idx_t per_user) {
+    if (role == R_SCR || role == R_SCR2) {
+      idx_t& s = P.scr_per_user[role - R_SCR];
+      if (per_user > s) s = per_user;
+    }
+  }
+
+  bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t) {
+    int CT = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
+    const PassVariant<T>* v = find_pass<T>(L, CT, colmap, epi, store_t);
+    if (!v) {
+      set_err("no kernel for L=%d colmap=%d epi=%d store_t=%d", L, colmap, epi, store_t);
+      ok = false;
+      return false;
+    }
+    r->type = PT_FFT;
+    r->L = L;
+    r->CT = CT;
+    r->colmap = colmap;
+    r->epi = epi;
+    r->store_t = store_t;
+    r->variant = v;
+    r->stage_tw = get_table<T>(0, L, 0);
+    if (!r->stage_tw) ok = false;
+    if (epi == EPI_R2C || epi == EPI_C2R) {
+      r->real_tw = L >= 2 ? get_table<T>(2, L, 0) : get_table<T>(2, 1, 0);
+      if (!r->real_tw) ok = false;
+    }
+    return ok;
+  }
+
+  // Fold user transforms into the column index when every side is a plain
+  // stack of the per-user tensor.
+  static bool can_fold(const Side& a, const Side& b, idx_t outer_u) {
+    auto f = [&](const Side& s) { return s.role >= R_SCR || s.s_u == outer_u * s.s_c; };
+    return f(a) && f(b);
+  }
+
+  void scratch_side(Side& s, int role) {
+    s = Side();
+    s.role = role;
+    s.il = 1;
+  }
+
+  // c2c along one axis of the per-user tensor (outer_u, n, inner); in and out
+  // carry role/format/offsets and s_u, strides are filled here.
+  void axis(Side in, Side out, idx_t outer_u, idx_t n, idx_t inner, int swi, int swo) {
+    if (!ok) return;
+    const idx_t tens = outer_u * n * inner;
+    need_scratch(in.role, tens);
+    need_scratch(out.role, tens);
+    if (in.role < R_SCR && in.s_u == 0) in.s_u = tens;
+    if (out.role < R_SCR && out.s_u == 0) out.s_u = tens;
+    const int tmp_role = (in.role == R_SCR || out.role == R_SCR) ? R_SCR2 : R_SCR;
+
+    if (inner == 1 && n <= max_row_len<T>()) {
+      PassRec* r = add();
+      if (!r || !fft_pass(r, (int)n, 0, EPI_PLAIN, 0)) return;
+      r->in = in;
+      r->out = out;
+      r->in.s_j = 1;
+      r->out.s_j = 1;
+      r->in.s_c = n;
+      r->out.s_c = n;
+      r->n_col = outer_u;
+      if (can_fold(r->in, r->out, outer_u)) {
+        r->fold_user = 1;
+      } else {
+        r->in.s_b = in.role >= R_SCR ? tens : in.s_u;
+        r->out.s_b = out.role >= R_SCR ? tens : out.s_u;
+        r->b_per_user = 1;
+      }
+      r->swap_in = swi;
+      r->swap_out = swo;
+      return;
+    }
+    // batch stride of a side whose batch index runs over (user, outer)
+    auto bstride = [&](const Side& s, idx_t per_outer) -> idx_t {
+      if (s.role < R_SCR && outer_u == 1) return s.s_u;
+      return per_outer;
+    };
+    if (outer_u > 1 && ((in.role < R_SCR && in.s_u != tens) || (out.role < R_SCR && out.s_u != tens))) {
+      set_err("internal: non-contiguous batch with outer axes");
+      ok = false;
+      return;
+    }
+    if (inner > 1 && n <= max_col_len<T>()) {
+      PassRec* r = add();
+      if (!r || !fft_pass(r, (int)n, 1, EPI_PLAIN, 0)) return;
+      r->in = in;
+      r->out = out;
+      r->in.s_j = inner;
+      r->out.s_j = inner;
+      r->in.s_c = 1;
+      r->out.s_c = 1;
+      r->in.s_b = bstride(in, n * inner);
+      r->out.s_b = bstride(out, n * inner);
+      r->n_col = inner;
+      r->b_per_user = outer_u;
+      r->swap_in = swi;
+      r->swap_out = swo;
+      return;
+    }
+    // four-step along the axis: n = n1 * n2
+    idx_t n2, n1;
+    if (inner == 1) {
+      n2 = n <= (idx_t)1024 * max_col_len<T>() ? 1024 : n / max_col_len<T>();
+      if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
+        set_err("transform length %lld is too long", (long long)n);
+        ok = false;
+        return;
+      }
+    } else {
+      n2 = max_col_len<T>();
+      if (n / n2 > max_col_len<T>()) {
+        set_err("axis length %lld is too long", (long long)n);
+        ok = false;
+        return;
+      }
+    }
+    n1 = n / n2;
+    if (n1 > max_col_len<T>() || n1 < 1) {
+      set_err("transform length %lld is too long", (long long)n);
+      ok = false;
+      return;
+    }
+    need_scratch(tmp_role, tens);
+    Side tA, tB;
+    scratch_side(tA, tmp_role);
+    scratch_side(tB, tmp_role);
+    const int bits = (ilog2(n) + 1) / 2;
+    const void* tw4 = get_table<T>(1, n, bits);
+    if (!tw4) {
+      ok = false;
+      return;
+    }
+    // pass A: length-n1 columns (stride n2*inner), twiddle W_n^(n2idx*k1)
+    PassRec* a = add();
+    if (!a || !fft_pass(a, (int)n1, 1, EPI_TW4, 0)) return;
+    a->in = in;
+    a->out = tA;
+    a->in.s_j = n2 * inner;
+    a->out.s_j = n2 * inner;
+    a->in.s_c = 1;
+    a->out.s_c = 1;
+    a->in.s_b = bstride(in, n * inner);
+    a->out.s_b = n * inner;
+    a->n_col = n2 * inner;
+    a->b_per_user = outer_u;
+    a->swap_in = swi;
+    a->tw4_hi = tw4;
+    a->tw4_lo = (const cx<T>*)tw4 + (n >> bits);
+    a->tw4_bits = bits;
+    a->tw4_shift = ilog2(inner);
+    // pass B: length-n2 transforms of every (k1, c); output bin k1 + n1*k2
+    PassRec* b = add();
+    if (inner == 1) {
+      if (!b || !fft_pass(b, (int)n2, 0, EPI_PLAIN, 1)) return;
+      b->in = tB;
+      b->out = out;
+      b->in.s_j = 1;
+      b->in.s_c = n2;
+      b->in.s_b = n;
+      b->out.s_j = n1;
+      b->out.s_c = 1;
+      b->out.s_b = bstride(out, n);
+      b->n_col = n1;
+      b->b_per_user = outer_u;
+      if (outer_u > 1) {
+        // outer index within a user transform, then the user index
+        b->n_blo = (int)outer_u;
+        b->in.s_b2 = n;
+        b->out.s_b2 = n;
+        b->in.s_b = outer_u * n;
+        b->out.s_b = (out.role >= R_SCR) ? outer_u * n : out.s_u;
+      }
+    } else {
+      if (!b || !fft_pass(b, (int)n2, 1, EPI_PLAIN, 0)) return;
+      b->in = tB;
+      b->out = out;
+      b->in.s_j = inner;
+      b->in.s_c = 1;
+      b->in.s_b = n * inner;
+      b->in.s_b2 = n2 * inner;
+      b->out.s_j = n1 * inner;
+      b->out.s_c = 1;
+      b->out.s_b = bstride(out, n * inner);
+      b->out.s_b2 = inner;
+      b->n_col = inner;
+      b->n_blo = (int)n1;
+      b->b_per_user = outer_u * n1;
+    }
+    b->swap_out = swo;
+  }
+
+  // c2c over every axis of the per-user tensor dims[0..nd): last axis first
+  // (reads `in`), then the outer axes in place on `out`.
+  void all_axes(Side in, Side out, const idx_t* d, int nd, int swi, int swo) {
+    idx_t total = 1;
+    for (int i = 0; i < nd; i++) total *= d[i];
+    Side cur = in;
+    for (int a = nd - 1; a >= 0; a--) {
+      idx_t outer = 1, inner = 1;
+      for (int i = 0; i < a; i++) outer *= d[i];
+      for (int i = a + 1; i < nd; i++) inner *= d[i];
+      if (d[a] == 1 && !(a == nd - 1 && nd == 1)) continue;
+      Side o = out;
+      axis(cur, o, outer, d[a], inner, swi, swo);
+      cur = out;
+      (void)total;
+    }
+  }
+
+  void real_rows(Side in, Side out, idx_t h, idx_t m, int inv) {
+    PassRec* r = add();
+    if (!r) return;
+    r->type = PT_REALROWS;
+    r->in = in;
+    r->out = out;
+    r->rr_h = h;
+    r->rr_m = m;
+    r->rr_inv = inv;
+    r->real_tw = get_table<T>(2, h, 0);
+    if (!r->real_tw) ok = false;
+  }
+
+  bool build(int kind, int format, const idx_t* d, int nd) {
+    const int swap = (kind == 1) ? 1 : 0;
+    idx_t total = 1;
+    for (int i = 0; i < nd; i++) total *= d[i];
+    Side usr_src, usr_dst;
+    usr_src.role = R_SRC;
+    usr_dst.role = R_DST;
+    usr_src.il = usr_dst.il = format == afft_interleaved;
+
+    if (kind == 0 || kind == 1) {
+      all_axes(usr_src, usr_dst, d, nd, swap, swap);
+      return ok;
+    }
+    // real transforms
+    if (total == 1) {
+      PassRec* r = add();
+      if (!r) return false;
+      r->type = PT_REALONE;
+      r->in = usr_src;
+      r->out = usr_dst;
+      r->rr_inv = kind == 3;
+      return ok;
+    }
+    const idx_t n0 = d[0], h = n0 / 2;
+    idx_t inner = 1;
+    for (int i = 1; i < nd; i++) inner *= d[i];
+    if (kind == 2) {
+      if (nd == 1) {
+        Side in = usr_src;
+        in.il = 1;  // N reals read as N/2 interleaved complex
+        in.s_u = h;
+        Side out = usr_dst;
+        out.s_u = h + 1;
+        if (h <= max_row_len<T>()) {
+          PassRec* r = add();
+          if (!r || !fft_pass(r, (int)h, 0, EPI_R2C, 0)) return false;
+          r->in = in;
+          r->out = out;
+          r->in.s_j = r->out.s_j = 1;
+          r->in.s_c = h;
+          r->out.s_c = h + 1;
+          r->n_col = 1;
+          r->fold_user = 1;
+          return ok;
+        }
+        axis(in, out, 1, h, 1, 0, 0);
+        Side rin = out, rout = out;
+        rin.s_j = rout.s_j = 1;
+        rin.s_b = rout.s_b = h + 1;
+        real_rows(rin, rout, h, 1, 0);
+        return ok;
+      }
+      if (h > max_col_len<T>()) {
+        set_err("real transforms with dim0/2 > %d are not supported for ndim > 1", max_col_len<T>());
+        return false;
+      }
+      PassRec* r = add();
+      if (!r || !fft_pass(r, (int)h, 1, EPI_R2C, 0)) return false;
+      r->in = usr_src;
+      r->in.il = 0;
+      r->in.one_array = 1;
+      r->in.re_off = 0;
+      r->in.im_off = inner;
+      r->in.s_j = 2 * inner;
+      r->in.s_c = 1;
+      r->in.s_b = r->in.s_u = n0 * inner;
+      r->out = usr_dst;
+      r->out.s_j = inner;
+      r->out.s_c = 1;
+      r->out.s_b = r->out.s_u = (h + 1) * inner;
+      r->n_col = inner;
+      r->b_per_user = 1;
+      // inner c2c on each of the h+1 rows, in place on dst
+      idx_t dd[kMaxDim];
+      dd[0] = h + 1;
+      for (int i = 1; i < nd; i++) dd[i] = d[i];
+      Side o = usr_dst;
+      o.s_u = (h + 1) * inner;
+      for (int a = nd - 1; a >= 1; a--) {
+        idx_t outer = 1, inn = 1;
+        for (int i = 0; i < a; i++) outer *= dd[i];
+        for (int i = a + 1; i < nd; i++) inn *= dd[i];
+        axis(o, o, outer, dd[a], inn, 0, 0);
+      }
+      return ok;
+    }
+    // kind == 3: c2r
+    if (nd == 1) {
+      Side in = usr_src;
+      in.s_u = h + 1;
+      Side out = usr_dst;
+      out.il = 1;  // N reals written as N/2 interleaved complex
+      out.s_u = h;
+      if (h <= max_row_len<T>()) {
+        PassRec* r = add();
+        if (!r || !fft_pass(r, (int)h, 0, EPI_C2R, 0)) return false;
+        r->in = in;
+        r->out = out;
+        r->in.s_j = r->out.s_j = 1;
+        r->in.s_c = h + 1;
+        r->out.s_c = h;
+        r->n_col = 1;
+        r->fold_user = 1;
+        r->swap_in = r->swap_out = 1;
+        return ok;
+      }
+      Side rin = in, rout = out;
+      rin.s_j = rout.s_j = 1;
+      rin.s_b = h + 1;
+      rout.s_b = h;
+      real_rows(rin, rout, h, 1, 1);
+      axis(out, out, 1, h, 1, 1, 1);
+      return ok;
+    }
+    if (h > max_col_len<T>()) {
+      set_err("real transforms with dim0/2 > %d are not supported for ndim > 1", max_col_len<T>());
+      return false;
+    }
+    idx_t dd[kMaxDim];
+    dd[0] = h + 1;
+    for (int i = 1; i < nd; i++) dd[i] = d[i];
+    Side in = usr_src;
+    in.s_u = (h + 1) * inner;
+    Side scr;
+    scratch_side(scr, R_SCR);
+    need_scratch(R_SCR, (h + 1) * inner);
+    Side cur = in;
+    for (int a = nd - 1; a >= 1; a--) {
+      idx_t outer = 1, inn = 1;
+      for (int i = 0; i < a; i++) outer *= dd[i];
+      for (int i = a + 1; i < nd; i++) inn *= dd[i];
+      axis(cur, scr, outer, dd[a], inn, 1, 1);
+      cur = scr;
+    }
+    PassRec* r = add();
+    if (!r || !fft_pass(r, (int)h, 1, EPI_C2R, 0)) return false;
+    r->in = scr;
+    r->in.s_j = inner;
+    r->in.s_c = 1;
+    r->in.s_b = (h + 1) * inner;
+    r->out = usr_dst;
+    r->out.il = 0;
+    r->out.one_array = 1;
+    r->out.re_off = 0;
+    r->out.im_off = inner;
+    r->out.s_j = 2 * inner;
+    r->out.s_c = 1;
+    r->out.s_b = r->out.s_u = n0 * inner;
+    r->n_col = inner;
+    r->b_per_user = 1;
+    r->swap_in = r->swap_out = 1;
+    return ok;
+  }
+};
+
+// ---- execution --------------------------------------------------------------
+template <typename T>
+struct Bufs {
+  const T* src_re;
+  const T* src_im;
+  T* dst_re;
+  T* dst_im;
+  T* scr[2];
+};
+
+template <typename T>
+void resolve(const Side& s, const Bufs<T>& B, idx_t start, const T*& re, const T*& im) {
+  const T* bre = nullptr;
+  const T* bim = nullptr;
+  idx_t uoff = 0;
+  switch (s.role) {
+    case R_SRC:
+      bre = B.src_re;
+      bim = B.src_im;
+      uoff = start * s.s_u;
+      break;
+    case R_DST:
+      bre = B.dst_re;
+      bim = B.dst_im;
+      uoff = start * s.s_u;
+      break;
+    default:
+      bre = B.scr[s.role - R_SCR];
+      bim = nullptr;
+      break;
+  }
+  if (s.il) {
+    re = bre + 2 * (s.re_off + uoff);
+    im = nullptr;
+  } else if (s.one_array) {
+    re = bre + s.re_off + uoff;
+    im = bre + s.im_off + uoff;
+  } else {
+    re = bre + s.re_off + uoff;
+    im = bim + s.im_off + uoff;
+  }
+}
+
+template <typename T>
+bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStream_t st) {
+  const T *ire, *iim, *ore_c, *oim_c;
+  resolve(r.in, B, start, ire, iim);
+  resolve(r.out, B, start, ore_c, oim_c);
+  T* ore = const_cast<T*>(ore_c);
+  T* oim = const_cast<T*>(oim_c);
+  if (r.type == PT_FFT) {
+    const PassVariant<T>* v = (const PassVariant<T>*)r.variant;
+    PassParams<T> p{};
+    p.in_re = ire;
+    p.in_im = iim;
+    p.out_re = ore;
+    p.out_im = oim;
+    p.in_js = r.in.s_j;
+    p.in_cs = r.in.s_c;
+    p.in_bs = r.in.s_b;
+    p.in_bs2 = r.in.s_b2;
+    p.out_ks = r.out.s_j;
+    p.out_cs = r.out.s_c;
+    p.out_bs = r.out.s_b;
+    p.out_bs2 = r.out.s_b2;
+    p.n_blo = r.n_blo;
+    idx_t ncol = r.fold_user ? r.n_col * cnt : r.n_col;
+    idx_t nb = r.fold_user ? 1 : r.b_per_user * cnt;
+    idx_t tiles = (ncol + r.CT - 1) / r.CT;
+    if (ncol > 0x7fffffff || tiles * nb > 0x7fffffff) {
+      set_err("launch too large");
+      return false;
+    }
+    p.n_col = (int)ncol;
+    p.col_tiles = (int)tiles;
+    p.n_tiles = (int)(tiles * nb);
+    p.in_il = r.in.il;
+    p.out_il = r.out.il;
+    p.swap_in = r.swap_in;
+    p.swap_out = r.swap_out;
+    p.stage_tw = (const cx<T>*)r.stage_tw;
+    p.tw4_hi = (const cx<T>*)r.tw4_hi;
+    p.tw4_lo = (const cx<T>*)r.tw4_lo;
+    p.tw4_lo_bits = r.tw4_bits;
+    p.tw4_col_shift = r.tw4_shift;
+    p.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(v->launch(p, st));
+    return true;
+  }
+  if (r.type == PT_REALROWS) {
+    RealRowsArgs<T> a{};
+    a.in_re = ire;
+    a.in_im = iim;
+    a.out_re = ore;
+    a.out_im = oim;
+    a.in_il = r.in.il;
+    a.out_il = r.out.il;
+    a.in_rs = r.in.s_j;
+    a.in_cs = r.in.s_c ? r.in.s_c : 1;
+    a.in_bs = r.in.s_b;
+    a.out_rs = r.out.s_j;
+    a.out_cs = r.out.s_c ? r.out.s_c : 1;
+    a.out_bs = r.out.s_b;
+    a.h = r.rr_h;
+    a.m = r.rr_m;
+    a.nbatch = cnt;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(real_rows<T>(a, r.rr_inv != 0, st));
+    return true;
+  }
+  // PT_REALONE
+  if (!r.rr_inv) {
+    HIP_OK(real_one<T>(ire, ore, oim, 0, r.out.il, cnt, 1, 1, st));
+  } else {
+    HIP_OK(real_one<T>(ire, ore, nullptr, 1, r.in.il, cnt, 1, 1, st));
+  }
+  return true;
+}
+
+template <typename T>
+bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
+             int last = -1) {
+  if (last < 0) last = P.npass;
+  Bufs<T> B{(const T*)sre, (const T*)sim, (T*)dre, (T*)dim, {(T*)P.scr[0], (T*)P.scr[1]}};
+  if (P.format == afft_interleaved) {
+    B.src_im = nullptr;
+    B.dst_im = nullptr;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+  for (idx_t start = 0; start < P.batch; start += P.chunk) {
+    idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
+    for (int i = first; i < last; i++)
+      if (!run_pass<T>(P.pass[i], B, start, cnt, st)) return false;
+  }
+  return true;
+}
+
+bool alloc_scratch(Plan& P, size_t elem_bytes) {
+  for (int i = 0; i < 2; i++) {
+    if (P.scr_per_user[i] == 0) continue;
+    size_t bytes = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes;
+    if (bytes == P.scr_bytes[i] && P.scr[i]) continue;
+    if (P.scr[i]) (void)hipFree(P.scr[i]);
+    P.scr[i] = nullptr;
+    P.scr_bytes[i] = 0;
+    hipError_t e = hipMalloc(&P.scr[i], bytes);
+    if (e != hipSuccess) {
+      set_err("scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      P.scr[i] = nullptr;
+      return false;
+    }
+    P.scr_bytes[i] = bytes;
+  }
+  return true;
+}
+
+idx_t default_chunk(const Plan& P, size_t elem_bytes) {
+  idx_t per = P.scr_per_user[0] > P.scr_per_user[1] ? P.scr_per_user[0] : P.scr_per_user[1];
+  if (per == 0) return P.batch;
+  // keep one pass group's scratch within ~64 MiB so it stays in the MALL
+  idx_t target = (idx_t)(64u << 20) / (idx_t)(per * elem_bytes);
+  if (target < 1) target = 1;
+  return target < P.batch ? target : P.batch;
+}
+
+// Validates arguments; returns squeezed dims in d / nd.  status: 0 ok,
+// 1 bad format, 2 other.
+int validate(size_t ndim, const size_t* dim, int format, int impl, idx_t* d, int& nd) {
+  if (format != afft_split && format != afft_interleaved) {
+    set_err("invalid format %d", format);
+    return 1;
+  }
+  if (impl != afft_auto && impl != afft_gpu) {
+    set_err("implementation %d is not provided by this library", impl);
+    return 2;
+  }
+  if (ndim > (size_t)kMaxDim) {
+    set_err("ndim %zu exceeds %d", ndim, kMaxDim);
+    return 2;
+  }
+  if (ndim > 0 && !dim) {
+    set_err("null dim array");
+    return 2;
+  }
+  nd = 0;
+  for (size_t i = 0; i < ndim; i++) {
+    if (!is_pow2((idx_t)dim[i])) {
+      set_err("dimension %zu (%zu) is not a power of two", i, dim[i]);
+      return 2;
+    }
+    if (dim[i] != 1) d[nd++] = (idx_t)dim[i];
+  }
+  if (nd == 0) d[nd++] = 1;
+  return 0;
+}
+
+template <typename T>
+void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t batch, void* mem, int impl) {
+  g_err.clear();
+  idx_t d[kMaxDim];
+  int nd = 0;
+  if (validate(ndim, dim, format, impl, d, nd) != 0) return nullptr;
+  if (!mem) {
+    set_err("null mem");
+    return nullptr;
+  }
+  if (batch < 1) {
+    set_err("batch must be >= 1");
+    return nullptr;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_err("no GPU visible");
+    return nullptr;
+  }
+  Plan* P = new (std::nothrow) Plan();
+  if (!P) {
+    set_err("out of host memory");
+    return nullptr;
+  }
+  P->prec = sizeof(T) == 4 ? 32 : 64;
+  P->kind = kind;
+  P->format = format;
+  P->ndim = nd;
+  for (int i = 0; i < nd; i++) P->dims[i] = d[i];
+  P->batch = (idx_t)batch;
+  (void)hipGetDevice(&P->device);
+  Builder<T> b(*P);
+  if (!b.build(kind, format, d, nd)) {
+    delete P;
+    return nullptr;
+  }
+  P->chunk = default_chunk(*P, sizeof(cx<T>));
+  if (!alloc_scratch(*P, sizeof(cx<T>))) {
+    delete P;
+    return nullptr;
+  }
+  Header h{kMagic, (uint32_t)P->prec, P};
+  std::memcpy(mem, &h, sizeof(h));
+  return mem;
+}
+
+void destroy_impl(void* state) {
+  if (!state) return;
+  Header h;
+  std::memcpy(&h, state, sizeof(h));
+  if (h.magic != kMagic || !h.plan) return;
+  Plan* P = h.plan;
+  for (int i = 0; i < 2; i++)
+    if (P->scr[i]) (void)hipFree(P->scr[i]);
+  delete P;
+  std::memset(state, 0, sizeof(h));
+}
+
+size_t memsize_impl(size_t ndim, const size_t* dim, int format, int impl) {
+  idx_t d[kMaxDim];
+  int nd = 0;
+  int st = validate(ndim, dim, format, impl, d, nd);
+  if (st == 1) return (size_t)-1;
+  if (st != 0) return 0;
+  return sizeof(Header);
+}
+
+template <typename T>
+void transform_impl(void* state, const void* sre, const void* sim, void* dre, void* dim, int kind) {
+  g_err.clear();
+  Header h;
+  if (!read_header(state, h)) return;
+  Plan& P = *h.plan;
+  if (P.prec != (int)(sizeof(T) * 8) || P.kind != kind) {
+    set_err("plan kind / precision mismatch");
+    return;
+  }
+  if (!execute<T>(P, sre, sim, dre, dim, P.stream)) return;
+  hipError_t e = hipStreamSynchronize(P.stream);
+  if (e != hipSuccess) set_err("transform: %s", hipGetErrorString(e));
+}
+
+}  // namespace
+
+// ---- stage schedule shared with the twiddle generator --------------------------
+namespace afgpu {
+template <int L>
+static int fill_stages(int* radix, int* ns) {
+  int acc = 1;
+  for (int s = 0; s < Sched<L>::NST; s++) {
+    radix[s] = Sched<L>::R[s];
+    ns[s] = acc;
+    acc *= Sched<L>::R[s];
+  }
+  return Sched<L>::NST;
+}
+int stage_list(int L, int* radix, int* ns) {
+  switch (L) {
+    case 1: return fill_stages<1>(radix, ns);
+    case 2: return fill_stages<2>(radix, ns);
+    case 4: return fill_stages<4>(radix, ns);
+    case 8: return fill_stages<8>(radix, ns);
+    case 16: return fill_stages<16>(radix, ns);
+    case 32: return fill_stages<32>(radix, ns);
+    case 64: return fill_stages<64>(radix, ns);
+    case 128: return fill_stages<128>(radix, ns);
+    case 256: return fill_stages<256>(radix, ns);
+    case 512: return fill_stages<512>(radix, ns);
+    case 1024: return fill_stages<1024>(radix, ns);
+    case 2048: return fill_stages<2048>(radix, ns);
+    case 4096: return fill_stages<4096>(radix, ns);
+    case 8192: return fill_stages<8192>(radix, ns);
+    case 16384: return fill_stages<16384>(radix, ns);
+    default: return 0;
+  }
+}
+int stage_tw_len(int L) {
+  int radix[8], ns[8];
+  int n = stage_list(L, radix, ns), tot = 0;
+  for (int s = 1; s < n; s++) tot += ns[s] * (radix[s] - 1);
+  return tot;
+}
+}  // namespace afgpu
+
+// ---- C ABI ----------------------------------------------------------------------
+extern "C" {
+
+const size_t afft_alignment = 64;
+
+size_t afft_32_align_size(size_t sz) { return (sz + 15) & ~(size_t)15; }
+size_t afft_64_align_size(size_t sz) { return (sz + 7) & ~(size_t)7; }
+
+int afft_supported(int impl) {
+  if (impl != afft_auto && impl != afft_gpu) return 0;
+  int n = 0;
+  return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+}
+
+#define AFFT_KIND_API(P, T, K, KID)                                                                    \
+  size_t afft_##P##_##K##_memsize(size_t ndim, const size_t* dim, int format, int impl) {              \
+    return memsize_impl(ndim, dim, format, impl);                                                     \
+  }                                                                                                   \
+  afft_##P##_##K##_type* afft_##P##_##K##_create(size_t ndim, const size_t* dim, int format, void* mem, \
+                                                 int impl) {                                          \
+    return (afft_##P##_##K##_type*)create_impl<T>(KID, ndim, dim, format, 1, mem, impl);              \
+  }                                                                                                   \
+  void afft_##P##_##K##_destroy(afft_##P##_##K##_type* state) { destroy_impl(state); }
+
+AFFT_KIND_API(32, float, c, 0)
+AFFT_KIND_API(32, float, ci, 1)
+AFFT_KIND_API(32, float, r, 2)
+AFFT_KIND_API(32, float, ri, 3)
+AFFT_KIND_API(64, double, c, 0)
+AFFT_KIND_API(64, double, ci, 1)
+AFFT_KIND_API(64, double, r, 2)
+AFFT_KIND_API(64, double, ri, 3)
+
+void afft_32_c_transform(afft_32_c_type* s, const float* sr, const float* si, float* dr, float* di) {
+  transform_impl<float>(s, sr, si, dr, di, 0);
+}
+void afft_32_ci_transform(afft_32_ci_type* s, const float* sr, const float* si, float* dr, float* di) {
+  transform_impl<float>(s, sr, si, dr, di, 1);
+}
+void afft_32_r_transform(afft_32_r_type* s, const float* src, float* dr, float* di) {
+  transform_impl<float>(s, src, nullptr, dr, di, 2);
+}
+void afft_32_ri_transform(afft_32_ri_type* s, const float* sr, const float* si, float* dst) {
+  transform_impl<float>(s, sr, si, dst, nullptr, 3);
+}
+void afft_64_c_transform(afft_64_c_type* s, const double* sr, const double* si, double* dr, double* di) {
+  transform_impl<double>(s, sr, si, dr, di, 0);
+}
+void afft_64_ci_transform(afft_64_ci_type* s, const double* sr, const double* si, double* dr, double* di) {
+  transform_impl<double>(s, sr, si, dr, di, 1);
+}
+void afft_64_r_transform(afft_64_r_type* s, const double* src, double* dr, double* di) {
+  transform_impl<double>(s, src, nullptr, dr, di, 2);
+}
+void afft_64_ri_transform(afft_64_ri_type* s, const double* sr, const double* si, double* dst) {
+  transform_impl<double>(s, sr, si, dst, nullptr, 3);
+}
+
+size_t afft_gpu_plan_memsize(void) { return sizeof(Header); }
+
+afft_gpu_plan* afft_gpu_plan_create(int precision, int kind, size_t ndim, const size_t* dim, int format,
+                                    size_t batch, void* mem) {
+  if (kind < 0 || kind > 3) {
+    set_err("invalid kind %d", kind);
+    return nullptr;
+  }
+  if (precision == 32) return (afft_gpu_plan*)create_impl<float>(kind, ndim, dim, format, batch, mem, afft_gpu);
+  if (precision == 64) return (afft_gpu_plan*)create_impl<double>(kind, ndim, dim, format, batch, mem, afft_gpu);
+  set_err("precision must be 32 or 64");
+  return nullptr;
+}
+
+int afft_gpu_execute(afft_gpu_plan* plan, const void* sre, const void* sim, void* dre, void* dim, void* stream) {
+  return afft_gpu_execute_range(plan, 0, -1, sre, sim, dre, dim, stream);
+}
+
+int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void* sre, const void* sim, void* dre,
+                           void* dim, void* stream) {
+  g_err.clear();
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  Plan& P = *h.plan;
+  if (last < 0 || last > P.npass) last = P.npass;
+  if (first < 0) first = 0;
+  bool ok = P.prec == 32 ? execute<float>(P, sre, sim, dre, dim, (hipStream_t)stream, first, last)
+                         : execute<double>(P, sre, sim, dre, dim, (hipStream_t)stream, first, last);
+  return ok ? 0 : -1;
+}
+
+int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass) {
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  const Plan& P = *h.plan;
+  if (pass < 0 || pass >= P.npass) return -1;
+  return (int)((P.batch + P.chunk - 1) / P.chunk);
+}
+
+void afft_gpu_plan_destroy(afft_gpu_plan* plan) { destroy_impl(plan); }
+
+int afft_gpu_set_stream(void* state, void* stream) {
+  Header h;
+  if (!read_header(state, h)) return -1;
+  h.plan->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk) {
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  Plan& P = *h.plan;
+  size_t eb = P.prec == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
+  idx_t c = chunk == 0 ? default_chunk(P, eb) : (idx_t)chunk;
+  if (c > P.batch) c = P.batch;
+  if (c < 1) c = 1;
+  idx_t old = P.chunk;
+  P.chunk = c;
+  if (!alloc_scratch(P, eb)) {
+    P.chunk = old;
+    alloc_scratch(P, eb);
+    return -1;
+  }
+  return 0;
+}
+
+int afft_gpu_plan_num_passes(const afft_gpu_plan* plan) {
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  return h.plan->npass;
+}
+
+size_t afft_gpu_plan_scratch_bytes(const afft_gpu_plan* plan) {
+  Header h;
+  if (!read_header(plan, h)) return 0;
+  return h.plan->scr_bytes[0] + h.plan->scr_bytes[1];
+}
+
+int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t len) {
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  const Plan& P = *h.plan;
+  if (i < 0 || i >= P.npass || !buf) return -1;
+  const PassRec& r = P.pass[i];
+  static const char* ep[] = {"plain", "tw4", "r2c", "c2r"};
+  if (r.type == PT_FFT)
+    snprintf(buf, len, "fft L=%d CT=%d %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
+             r.L, r.CT, r.colmap ? "col" : "row", ep[r.epi], r.store_t ? " store_t" : "", (long long)r.n_col,
+             (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
+  else if (r.type == PT_REALROWS)
+    snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
+  else
+    snprintf(buf, len, "real_one inv=%d", r.rr_inv);
+  return 0;
+}
+
+const char* afft_gpu_last_error(void) { return g_err.c_str(); }
+
+static void fill_side(const Side& s, afft_gpu_side_info& o) {
+  o.role = s.role;
+  o.il = s.il;
+  o.one_array = s.one_array;
+  o.re_off = s.re_off;
+  o.im_off = s.im_off;
+  o.s_j = s.s_j;
+  o.s_c = s.s_c;
+  o.s_b = s.s_b;
+  o.s_b2 = s.s_b2;
+  o.s_u = s.s_u;
+}
+
+int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                          afft_gpu_plan_info* info, afft_gpu_pass_info* passes, int max_passes) {
+  idx_t d[kMaxDim];
+  int nd = 0;
+  if (validate(ndim, dim, format, afft_gpu, d, nd) != 0) return -1;
+  if (batch < 1 || kind < 0 || kind > 3 || (precision != 32 && precision != 64)) {
+    set_err("invalid arguments");
+    return -1;
+  }
+  Plan* P = new (std::nothrow) Plan();
+  if (!P) return -1;
+  P->prec = precision;
+  P->kind = kind;
+  P->format = format;
+  P->ndim = nd;
+  for (int i = 0; i < nd; i++) P->dims[i] = d[i];
+  P->batch = (idx_t)batch;
+  g_dry = true;
+  bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)
+                            : Builder<double>(*P).build(kind, format, d, nd);
+  g_dry = false;
+  if (!ok) {
+    delete P;
+    return -1;
+  }
+  size_t eb = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
+  P->chunk = default_chunk(*P, eb);
+  if (info) {
+    info->npass = P->npass;
+    info->chunk = P->chunk;
+    info->scr_per_user[0] = P->scr_per_user[0];
+    info->scr_per_user[1] = P->scr_per_user[1];
+  }
+  int n = P->npass < max_passes ? P->npass : max_passes;
+  for (int i = 0; i < n && passes; i++) {
+    const PassRec& r = P->pass[i];
+    afft_gpu_pass_info& o = passes[i];
+    std::memset(&o, 0, sizeof(o));
+    o.type = r.type;
+    o.L = r.L;
+    o.CT = r.CT;
+    o.colmap = r.colmap;
+    o.epi = r.epi;
+    o.store_t = r.store_t;
+    fill_side(r.in, o.in);
+    fill_side(r.out, o.out);
+    o.n_col = r.n_col;
+    o.fold_user = r.fold_user;
+    o.b_per_user = r.b_per_user;
+    o.n_blo = r.n_blo;
+    o.swap_in = r.swap_in;
+    o.swap_out = r.swap_out;
+    o.tw4_bits = r.tw4_bits;
+    o.tw4_shift = r.tw4_shift;
+    o.rr_h = r.rr_h;
+    o.rr_m = r.rr_m;
+    o.rr_inv = r.rr_inv;
+  }
+  int np = P->npass;
+  delete P;
+  return np;
+}
+
+int afft_gpu_error_sums_f32(const float* y, const double* ref, size_t n, double* acc, void* stream) {
+  return err_sums<float>(y, ref, (idx_t)n, acc, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+int afft_gpu_error_sums_f64(const double* y, const double* ref, size_t n, double* acc, void* stream) {
+  return err_sums<double>(y, ref, (idx_t)n, acc, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+int afft_gpu_copy(const void* src, void* dst, size_t bytes, void* stream) {
+  return copy16(src, dst, (idx_t)bytes, (hipStream_t)stream) == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

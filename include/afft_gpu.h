@@ -1,0 +1,132 @@
+/*
+ * afft_gpu.h -- extensions of the MI355X afft backend beyond fft.h.
+ *
+ * None of these has a counterpart in the reference (afft has no batch, stream,
+ * destroy, precision mode or work-queue API).  They exist for callers that
+ * need asynchronous, batched or mixed-size execution on the GPU.  Every
+ * pointer argument named src/dst is a device pointer; `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef AFFT_MI355X_GPU_H
+#define AFFT_MI355X_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fft.h"
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/* impl id of this backend; afft_auto selects it too */
+enum { afft_gpu = 16 };
+
+/* transform kinds */
+enum { afft_gpu_c = 0, afft_gpu_ci = 1, afft_gpu_r = 2, afft_gpu_ri = 3 };
+
+typedef struct afft_gpu_plan_struct afft_gpu_plan;
+
+/* Bytes of host memory a plan header needs (any alignment). */
+size_t afft_gpu_plan_memsize(void);
+
+/* A batch of `batch` transforms of shape dim[0..ndim) laid out back to back:
+ *   c / ci : N complex in, N complex out per transform;
+ *   r      : N reals in, (dim0/2+1)*inner complex out per transform;
+ *   ri     : (dim0/2+1)*inner complex in, N reals out per transform;
+ * (split format: each of the re / im arrays advances by the per-transform
+ * element count).  precision is 32 or 64.  Returns NULL on error
+ * (afft_gpu_last_error tells why). */
+afft_gpu_plan* afft_gpu_plan_create(int precision, int kind, size_t ndim, const size_t* dim, int format,
+                                    size_t batch, void* mem);
+
+/* Enqueues the transform on `stream` and returns without waiting.  Returns 0
+ * on success, -1 on error. */
+int afft_gpu_execute(afft_gpu_plan* plan, const void* src_re, const void* src_im, void* dst_re, void* dst_im,
+                     void* stream);
+
+/* Enqueues only passes [first, last) of the plan (last = -1: to the end),
+ * for every chunk of the batch.  For measuring one kernel of a multi-pass
+ * plan; running the ranges in order equals afft_gpu_execute. */
+int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void* src_re, const void* src_im,
+                           void* dst_re, void* dst_im, void* stream);
+
+/* Kernel launches pass `pass` makes per execute (one per chunk of the
+ * batch), or -1. */
+int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass);
+
+/* Releases the plan's device resources (the header memory stays the
+ * caller's). */
+void afft_gpu_plan_destroy(afft_gpu_plan* plan);
+
+/* Works for a plan made by any afft_XX_Y_create of fft.h as well: the stream
+ * its transform call runs on (and then waits for). */
+int afft_gpu_set_stream(void* state, void* stream);
+
+/* Number of transforms processed per pass group in large four-step plans
+ * (0 = automatic).  Tuning knob; the result does not depend on it. */
+int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk);
+
+/* Plan introspection: number of kernel launches per execute, and the scratch
+ * bytes held on the device. */
+int afft_gpu_plan_num_passes(const afft_gpu_plan* plan);
+size_t afft_gpu_plan_scratch_bytes(const afft_gpu_plan* plan);
+/* Writes a one-line description of pass i (kernel shape) into buf. */
+int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t len);
+
+/* The last error message of the calling thread ("" if none). */
+const char* afft_gpu_last_error(void);
+
+/* ---- plan inspection (host only, no GPU needed) ---------------------------- */
+/* Buffer side of a pass: role 0 = src, 1 = dst, 2/3 = plan scratch.  il = 1:
+ * interleaved complex addressing (offsets/strides in complex elements); il = 0:
+ * split addressing in scalar elements, with im taken from the re array when
+ * one_array = 1.  Element (b, c, j) of a pass is at
+ * re_off + s_u*user + (b / n_blo)*s_b + (b % n_blo)*s_b2 + c*s_c + j*s_j. */
+typedef struct
+{
+  int32_t role, il, one_array, pad_;
+  int64_t re_off, im_off, s_j, s_c, s_b, s_b2, s_u;
+} afft_gpu_side_info;
+
+typedef struct
+{
+  int32_t type; /* 0 = FFT pass, 1 = real-rows pass, 2 = single-value real */
+  int32_t L, CT, colmap, epi, store_t;
+  afft_gpu_side_info in, out;
+  int64_t n_col;
+  int32_t fold_user, n_blo;
+  int64_t b_per_user;
+  int32_t swap_in, swap_out, tw4_bits, tw4_shift;
+  int64_t rr_h, rr_m;
+  int32_t rr_inv, pad_;
+} afft_gpu_pass_info;
+
+typedef struct
+{
+  int32_t npass, pad_;
+  int64_t chunk;
+  int64_t scr_per_user[2];
+} afft_gpu_plan_info;
+
+/* Builds the pass list of a plan on the host only (no GPU calls, nothing
+ * allocated on the device) and copies up to max_passes descriptors out.
+ * Returns the number of passes, or -1 on error.  For tests and tooling that
+ * check a plan's addressing without a GPU. */
+int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                          afft_gpu_plan_info* info, afft_gpu_pass_info* passes, int max_passes);
+
+/* ---- helpers -------------------------------------------------------------- */
+/* acc[0] = sum (y[i] - ref[i])^2, acc[1] = sum ref[i]^2 over n values, in
+ * FP64 on the device (acc is a device array of 2 doubles). */
+int afft_gpu_error_sums_f32(const float* y, const double* ref, size_t n, double* acc, void* stream);
+int afft_gpu_error_sums_f64(const double* y, const double* ref, size_t n, double* acc, void* stream);
+/* Streaming device copy (16-byte accesses); the measured bandwidth ceiling. */
+int afft_gpu_copy(const void* src, void* dst, size_t bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

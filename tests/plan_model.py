@@ -1,0 +1,264 @@
+"""A numpy model of the GPU plans, for CPU tests.
+
+``afft_gpu_plan_inspect`` returns the pass list the library would launch.  This
+module executes that list on host arrays using exactly the addressing the
+kernels use (fft_amd/csrc/afft_pass.h, afft_aux.hip): every element address of
+every pass is computed, bounds-checked against the buffer sizes the API
+contract gives the caller's arrays and the plan's scratch, gathered, pushed
+through the pass's arithmetic (numpy FFT in float64 plus the pass epilogue) and
+scattered.  The result is compared with the oracle by the tests, so a wrong
+stride, role, chunk offset or epilogue in the plan builder fails on the CPU
+before any kernel runs on a GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from fft_amd import _lib
+
+
+class SideInfo(ctypes.Structure):
+    _fields_ = [("role", ctypes.c_int32), ("il", ctypes.c_int32), ("one_array", ctypes.c_int32),
+                ("pad_", ctypes.c_int32), ("re_off", ctypes.c_int64), ("im_off", ctypes.c_int64),
+                ("s_j", ctypes.c_int64), ("s_c", ctypes.c_int64), ("s_b", ctypes.c_int64),
+                ("s_b2", ctypes.c_int64), ("s_u", ctypes.c_int64)]
+
+
+class PassInfo(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("L", ctypes.c_int32), ("CT", ctypes.c_int32),
+                ("colmap", ctypes.c_int32), ("epi", ctypes.c_int32), ("store_t", ctypes.c_int32),
+                ("in_", SideInfo), ("out", SideInfo), ("n_col", ctypes.c_int64),
+                ("fold_user", ctypes.c_int32), ("n_blo", ctypes.c_int32), ("b_per_user", ctypes.c_int64),
+                ("swap_in", ctypes.c_int32), ("swap_out", ctypes.c_int32), ("tw4_bits", ctypes.c_int32),
+                ("tw4_shift", ctypes.c_int32), ("rr_h", ctypes.c_int64), ("rr_m", ctypes.c_int64),
+                ("rr_inv", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("npass", ctypes.c_int32), ("pad_", ctypes.c_int32), ("chunk", ctypes.c_int64),
+                ("scr_per_user", ctypes.c_int64 * 2)]
+
+
+KIND = {"c": 0, "ci": 1, "r": 2, "ri": 3}
+
+
+def inspect(kind: str, dims, fmt: int, batch: int = 1, prec: int = 32):
+    L = _lib.lib()
+    f = L.afft_gpu_plan_inspect
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
+                  ctypes.c_size_t, ctypes.POINTER(PlanInfo), ctypes.POINTER(PassInfo), ctypes.c_int]
+    f.restype = ctypes.c_int
+    nd, arr = _lib.dims_array(dims)
+    info = PlanInfo()
+    passes = (PassInfo * 200)()
+    n = f(prec, KIND[kind], nd, arr, fmt, batch, ctypes.byref(info), passes, 200)
+    if n < 0:
+        raise ValueError(_lib.last_error())
+    return info, [passes[i] for i in range(n)]
+
+
+def squeeze(dims):
+    s = [int(d) for d in dims if int(d) != 1]
+    return s or [1]
+
+
+class Buffers:
+    """Scalar float64 arrays standing for the device buffers of one execute."""
+
+    def __init__(self, kind, dims, fmt, batch, info):
+        d = squeeze(dims)
+        n = int(np.prod(d))
+        hsz = (d[0] // 2 + 1) * int(np.prod(d[1:])) if kind in ("r", "ri") else n
+        il = fmt == _lib.afft_interleaved
+        cx_in = kind in ("c", "ci", "ri")
+        cx_out = kind in ("c", "ci", "r")
+        nin = n if kind != "ri" else hsz
+        nout = n if kind != "r" else hsz
+        self.arrays = {}
+        self.arrays["src_re"] = np.zeros((2 if (cx_in and il) else 1) * nin * batch)
+        self.arrays["src_im"] = np.zeros(nin * batch if (cx_in and not il) else 0)
+        self.arrays["dst_re"] = np.zeros((2 if (cx_out and il) else 1) * nout * batch)
+        self.arrays["dst_im"] = np.zeros(nout * batch if (cx_out and not il) else 0)
+        for i in range(2):
+            self.arrays[f"scr{i}"] = np.zeros(2 * info.scr_per_user[i] * info.chunk)
+
+
+def _arrays_for(side, bufs):
+    r = side.role
+    if r == 0:
+        return bufs.arrays["src_re"], bufs.arrays["src_im"]
+    if r == 1:
+        return bufs.arrays["dst_re"], bufs.arrays["dst_im"]
+    return bufs.arrays[f"scr{r - 2}"], None
+
+
+def _addr(side, start, b, n_blo, c, j):
+    """Element index (complex units when il, scalar units otherwise)."""
+    uoff = start * side.s_u if side.role < 2 else 0
+    return (uoff + (b // n_blo) * side.s_b + (b % n_blo) * side.s_b2 + c * side.s_c + j * side.s_j)
+
+
+def _check(idx, arr, what):
+    if idx.size == 0:
+        return
+    lo, hi = int(idx.min()), int(idx.max())
+    if lo < 0 or hi >= arr.size:
+        raise IndexError(f"{what}: index range [{lo}, {hi}] outside buffer of {arr.size}")
+
+
+def _gather(side, bufs, e, what):
+    re_arr, im_arr = _arrays_for(side, bufs)
+    if side.il:
+        i0 = 2 * (side.re_off + e)
+        _check(i0, re_arr, what + " (re)")
+        _check(i0 + 1, re_arr, what + " (im)")
+        return re_arr[i0] + 1j * re_arr[i0 + 1]
+    ire = side.re_off + e
+    iim = side.im_off + e
+    src_im = re_arr if side.one_array else im_arr
+    _check(ire, re_arr, what + " (re)")
+    _check(iim, src_im, what + " (im)")
+    return re_arr[ire] + 1j * src_im[iim]
+
+
+def _scatter(side, bufs, e, v, what):
+    re_arr, im_arr = _arrays_for(side, bufs)
+    if side.il:
+        i0 = 2 * (side.re_off + e)
+        _check(i0, re_arr, what + " (re)")
+        _check(i0 + 1, re_arr, what + " (im)")
+        re_arr[i0] = v.real
+        re_arr[i0 + 1] = v.imag
+        return
+    ire = side.re_off + e
+    iim = side.im_off + e
+    dst_im = re_arr if side.one_array else im_arr
+    _check(ire, re_arr, what + " (re)")
+    _check(iim, dst_im, what + " (im)")
+    re_arr[ire] = v.real
+    dst_im[iim] = v.imag
+
+
+def _swap(z):
+    return z.imag + 1j * z.real
+
+
+def _fft_pass(p, bufs, start, cnt, pi):
+    L = p.L
+    ncol = p.n_col * cnt if p.fold_user else p.n_col
+    nb = 1 if p.fold_user else p.b_per_user * cnt
+    b = np.arange(nb).reshape(nb, 1, 1)
+    c = np.arange(ncol).reshape(1, ncol, 1)
+    nin = L + 1 if p.epi == 3 else L
+    j = np.arange(nin).reshape(1, 1, nin)
+    e_in = _addr(p.in_, start, b, p.n_blo, c, j)
+    x = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    if p.epi == 3:  # c2r pre-pass
+        s0 = x[..., :L]
+        s1 = x[..., L:0:-1]  # S[L - k], k = 0..L-1
+        k = np.arange(L)
+        w = np.exp(-2j * np.pi * k / (2 * L))
+        a = s0 + np.conj(s1)
+        bb = (np.conj(s1) - s0) * np.conj(w)
+        z = a - 1j * bb
+        z[..., 0] = (s0[..., 0].real + s1[..., 0].real) + 1j * (s0[..., 0].real - s1[..., 0].real)
+        x = z
+    if p.swap_in:
+        x = _swap(x)
+    X = np.fft.fft(x, axis=-1)
+    if p.epi == 1:  # four-step twiddle
+        n2 = (np.arange(ncol) >> p.tw4_shift).reshape(1, ncol, 1)
+        N = L * (p.n_col >> p.tw4_shift)
+        k1 = np.arange(L).reshape(1, 1, L)
+        X = X * np.exp(-2j * np.pi * ((n2 * k1) % N) / N)
+    if p.epi == 2:  # r2c post-pass, bins 0..L
+        Z = X
+        k = np.arange(L)
+        Zh = Z[..., (-k) % L]
+        w = np.exp(-2j * np.pi * k / (2 * L))
+        a = 0.5 * (Z + np.conj(Zh))
+        bb = 0.5 * (Z - np.conj(Zh)) * w
+        out = a - 1j * bb
+        out[..., 0] = Z[..., 0].real + Z[..., 0].imag
+        last = (Z[..., 0].real - Z[..., 0].imag)[..., None]
+        X = np.concatenate([out, last], axis=-1)
+    if p.swap_out:
+        X = _swap(X)
+    kk = np.arange(X.shape[-1]).reshape(1, 1, X.shape[-1])
+    e_out = _addr(p.out, start, b, p.n_blo, c, kk)
+    # the kernels never write the same element twice within a pass
+    flat = np.broadcast_to(e_out, X.shape).ravel()
+    if np.unique(flat).size != flat.size:
+        raise AssertionError(f"pass {pi}: overlapping output addresses")
+    _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
+
+
+def _real_rows(p, bufs, start, cnt, pi):
+    h, m = p.rr_h, p.rr_m
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    c = np.arange(m).reshape(1, m, 1)
+    rows = np.arange(h + 1).reshape(1, 1, h + 1)
+    sc_in = p.in_.s_c or 1
+    sc_out = p.out.s_c or 1
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    if p.rr_inv:
+        e_in = uin + b * p.in_.s_b + c * sc_in + rows * p.in_.s_j
+        S = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+        k = np.arange(h)
+        s0 = S[..., :h]
+        s1 = S[..., h - k]
+        w = np.exp(-2j * np.pi * k / (2 * h))
+        a = s0 + np.conj(s1)
+        bb = (np.conj(s1) - s0) * np.conj(w)
+        Z = a - 1j * bb
+        Z[..., 0] = (S[..., 0].real + S[..., h].real) + 1j * (S[..., 0].real - S[..., h].real)
+        orows = np.arange(h).reshape(1, 1, h)
+        e_out = uout + b * p.out.s_b + c * sc_out + orows * p.out.s_j
+        _scatter(p.out, bufs, e_out, Z, f"pass {pi} store")
+    else:
+        hr = np.arange(h).reshape(1, 1, h)
+        e_in = uin + b * p.in_.s_b + c * sc_in + hr * p.in_.s_j
+        Z = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+        k = np.arange(h)
+        Zh = Z[..., (-k) % h]
+        w = np.exp(-2j * np.pi * k / (2 * h))
+        X = 0.5 * (Z + np.conj(Zh)) - 1j * 0.5 * (Z - np.conj(Zh)) * w
+        X[..., 0] = Z[..., 0].real + Z[..., 0].imag
+        last = (Z[..., 0].real - Z[..., 0].imag)[..., None]
+        X = np.concatenate([X, last], axis=-1)
+        e_out = uout + b * p.out.s_b + c * sc_out + rows * p.out.s_j
+        _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
+
+
+def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32):
+    """Executes the plan model; returns the dst arrays (float64)."""
+    info, passes = inspect(kind, dims, fmt, batch, prec)
+    bufs = Buffers(kind, dims, fmt, batch, info)
+    if bufs.arrays["src_re"].size != np.asarray(src_re).size:
+        raise ValueError("src_re size does not match the API contract")
+    bufs.arrays["src_re"][:] = np.asarray(src_re, dtype=np.float64).ravel()
+    if src_im is not None:
+        bufs.arrays["src_im"][:] = np.asarray(src_im, dtype=np.float64).ravel()
+    for start in range(0, batch, info.chunk):
+        cnt = min(info.chunk, batch - start)
+        for pi, p in enumerate(passes):
+            if p.type == 0:
+                _fft_pass(p, bufs, start, cnt, pi)
+            elif p.type == 1:
+                _real_rows(p, bufs, start, cnt, pi)
+            else:
+                for u in range(start, start + cnt):
+                    if not p.rr_inv:
+                        v = bufs.arrays["src_re"][u]
+                        if p.out.il:
+                            bufs.arrays["dst_re"][2 * u:2 * u + 2] = [v, 0.0]
+                        else:
+                            bufs.arrays["dst_re"][u] = v
+                            bufs.arrays["dst_im"][u] = 0.0
+                    else:
+                        s = bufs.arrays["src_re"]
+                        bufs.arrays["dst_re"][u] = s[2 * u] if p.in_.il else s[u]
+    return bufs.arrays["dst_re"], bufs.arrays["dst_im"], info, passes

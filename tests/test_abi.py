@@ -1,0 +1,165 @@
+"""CPU tests of the C ABI library: it loads, exports every symbol the headers
+declare, keeps afft's memsize / create conventions, and every plan's pass list
+(built host-only by afft_gpu_plan_inspect) computes the right transform when
+executed by the numpy plan model with all accesses in bounds.  No compute call
+reaches a GPU here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fft_amd
+from fft_amd import _lib
+from oracle import oracle as O
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("fft.h", "afft_gpu.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(afft_\w+)\s*\(", txt):
+            names.add(m.group(1))
+        for m in re.finditer(r"extern\s+const\s+size_t\s+(afft_\w+)", txt):
+            names.add(m.group(1))
+    return names
+
+
+def exported_symbols():
+    out = os.popen(f"nm -D --defined-only {_lib.LIB_PATH}").read()
+    return {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = fft_amd.lib()
+    assert L is not None
+    decl = declared_symbols()
+    assert len(decl) >= 40
+    missing = decl - exported_symbols()
+    assert not missing, missing
+    for s in _lib.FFT_H_SYMBOLS + _lib.AFFT_GPU_H_SYMBOLS:
+        assert s in decl, s
+
+
+def test_constants_and_align():
+    L = fft_amd.lib()
+    assert ctypes.c_size_t.in_dll(L, "afft_alignment").value == 64
+    # afft_32_align_size rounds up to 64 bytes of floats (fft.cpp:219)
+    assert [L.afft_32_align_size(n) for n in (0, 1, 16, 17, 33)] == [0, 16, 16, 32, 48]
+    assert [L.afft_64_align_size(n) for n in (0, 1, 8, 9)] == [0, 8, 8, 16]
+    # CPU ISA ids are not provided by this library (afft: unsupported -> 0)
+    for impl in (1, 2, 3, 4, 5):
+        assert L.afft_supported(impl) == 0
+
+
+def test_memsize_conventions():
+    L = fft_amd.lib()
+    nd, dims = _lib.dims_array([4096])
+    for p in ("32", "64"):
+        for k in ("c", "ci", "r", "ri"):
+            ms = getattr(L, f"afft_{p}_{k}_memsize")
+            assert ms(nd, dims, 1, 0) > 0
+            assert ms(nd, dims, 0, 16) > 0  # afft_gpu
+            assert ms(nd, dims, 7, 0) == ctypes.c_size_t(-1).value  # invalid format
+            assert ms(nd, dims, 1, 3) == 0  # afft_avx2: not provided here
+            nd3, d3 = _lib.dims_array([12])
+            assert ms(nd3, d3, 1, 0) == 0  # not a power of two
+
+
+def test_create_rejects_without_crashing():
+    L = fft_amd.lib()
+    nd, dims = _lib.dims_array([8])
+    mem = ctypes.create_string_buffer(256)
+    assert not L.afft_32_c_create(nd, dims, 5, mem, 0)  # invalid format
+    assert not L.afft_32_c_create(nd, dims, 1, mem, 4)  # CPU ISA
+    # a null state is a no-op for destroy
+    L.afft_32_c_destroy(None)
+
+
+# ---- plan model -----------------------------------------------------------------
+
+import plan_model as M  # noqa: E402
+
+IL, SP = _lib.afft_interleaved, _lib.afft_split
+
+
+def _pack(x, fmt):
+    x = np.asarray(x).reshape(-1)
+    if fmt == IL:
+        return np.stack([x.real, x.imag], -1).ravel(), None
+    return x.real.copy(), x.imag.copy()
+
+
+def _unpack(re_, im_, fmt):
+    return re_[0::2] + 1j * re_[1::2] if fmt == IL else re_ + 1j * im_
+
+
+def model_check(kind, dims, fmt, batch, prec=32):
+    s = O.squeeze_shape(dims)
+    n = int(np.prod(s))
+    if kind in ("c", "ci"):
+        x = O.complex_input(1, (batch, n)).astype(np.complex128)
+        re_, im_ = _pack(x, fmt)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec)
+        y = _unpack(dre, dim_, fmt).reshape(batch, n)
+        ref = np.stack([O.fft(x[b].reshape(s), inverse=(kind == "ci")).ravel() for b in range(batch)])
+    elif kind == "r":
+        x = O.normal(2, batch * n).astype(np.float64).reshape(batch, n)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, x, None, prec)
+        y = _unpack(dre, dim_, fmt).reshape(batch, -1)
+        ref = np.stack([O.rfft(x[b].reshape(s)).ravel() for b in range(batch)])
+    else:
+        hs = O.half_shape(s)
+        z = np.stack([O.hermitian_input(3 + b, s).astype(np.complex128).ravel() for b in range(batch)])
+        re_, im_ = _pack(z, fmt)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec)
+        y = dre.reshape(batch, n)
+        ref = np.stack([O.irfft(z[b].reshape(hs), s).ravel() for b in range(batch)])
+    return np.abs(y - ref).max() / max(1e-300, np.abs(ref).max()), info, passes
+
+
+SHAPES = [(1,), (2,), (4,), (8,), (32,), (64,), (4096,), (16384,), (32768,), (1 << 17,), (16, 32), (8, 8, 8),
+          (2, 4), (4, 1, 8), (8192, 4), (2, 2, 2, 2), (32, 16384), (4, 32768)]
+
+
+@pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])
+@pytest.mark.parametrize("dims", SHAPES, ids=lambda d: "x".join(map(str, d)))
+def test_plan_model_matches_oracle(kind, dims):
+    for fmt in (IL, SP):
+        for batch in (1, 3):
+            err, info, passes = model_check(kind, dims, fmt, batch)
+            assert err < 1e-9, (kind, dims, fmt, batch, err)
+
+
+def test_plan_model_fp64_and_chunking():
+    # a batch larger than one chunk exercises the per-chunk scratch reuse
+    info, passes = M.inspect("c", (1 << 20,), IL, 64)
+    assert len(passes) == 2 and 1 <= info.chunk < 64
+    err, info, _ = model_check("c", (1 << 15,), IL, 3, prec=64)
+    assert err < 1e-9
+
+
+def test_plan_shapes_of_headline_configs():
+    # 64 x 2^20 c2c: two passes (four-step 1024 x 1024)
+    info, passes = M.inspect("c", (1 << 20,), IL, 64)
+    assert [(p.L, p.colmap, p.epi, p.store_t) for p in passes] == [(1024, 1, 1, 0), (1024, 0, 0, 1)]
+    # 2^12 c2c: one row pass
+    info, passes = M.inspect("c", (4096,), IL, 1)
+    assert [(p.L, p.colmap) for p in passes] == [(4096, 0)]
+    # 8192^2 r2c: fused dim-0 column r2c, then the 8192-point rows
+    info, passes = M.inspect("r", (8192, 8192), IL, 1)
+    assert [(p.L, p.colmap, p.epi) for p in passes] == [(4096, 1, 2), (8192, 0, 0)]
+    # 256^3 c2c: three single passes
+    info, passes = M.inspect("c", (256, 256, 256), IL, 1)
+    assert len(passes) == 3
+
+
+def test_unsupported_shapes_report_errors():
+    with pytest.raises(ValueError):
+        M.inspect("r", (65536, 2), IL, 1)
+    with pytest.raises(ValueError):
+        M.inspect("c", (12,), IL, 1)

@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle and
+afft's own output (golden fixtures written from the reference build).
+
+Bar: rel-L2 <= 2*log2(N)*2^-24 against the FP64 oracle (N = total element
+count) and against afft's FP32 output on the same seeded input; FP64 plans
+must sit within 2*log2(N)*2^-53 * 16 of the oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def G():
+    import gpu_util
+    gpu_util.torch_cuda()
+    return gpu_util
+
+
+def bound64(n):
+    return 16 * 2.0 * np.log2(max(2, n)) * 2.0 ** -53
+
+
+@pytest.mark.parametrize("lg", list(range(0, 23)))
+def test_c2c_1d(G, lg):
+    n = 1 << lg
+    x = O.complex_input(1, (n,))
+    ref_f = O.fft(x.astype(np.complex128))
+    ref_i = O.fft(x.astype(np.complex128), inverse=True)
+    for fmt in (G.IL, G.SP):
+        say("c2c 1d", n, "fmt", fmt)
+        y = G.gpu_c2c(x, False, fmt)
+        assert O.rel_l2(y.astype(np.complex64), ref_f) <= O.bound(n), (n, fmt)
+        yi = G.gpu_c2c(x, True, fmt)
+        assert O.rel_l2(yi.astype(np.complex64), ref_i) <= O.bound(n), (n, fmt)
+
+
+def test_c2c_against_afft_golden(G):
+    g = np.load(os.path.join(GOLD, "golden_small.npz"), allow_pickle=False)
+    checked = 0
+    for k in g.files:
+        kind, shape, fmt, isa = k.split("|")
+        shape = tuple(int(s) for s in shape.split("x"))
+        n = int(np.prod(shape))
+        f = G.IL if fmt == "il" else G.SP
+        say("golden", k)
+        if kind in ("c", "ci"):
+            y = G.gpu_c2c(O.complex_input(1, shape), kind == "ci", f)
+        elif kind == "r":
+            y = G.gpu_r2c(O.uniform(2, n).reshape(shape), f)
+        else:
+            y = G.gpu_c2r(O.hermitian_input(3, shape), shape, f)
+        gold = g[k]
+        ref = gold.astype(np.complex128) if np.iscomplexobj(gold) else gold.astype(np.float64)
+        y32 = y.astype(np.complex64) if np.iscomplexobj(y) else y.astype(np.float32)
+        assert O.rel_l2(y32, ref) <= O.bound(n), (k, O.rel_l2(y32, ref))
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("lg", [1, 2, 3, 4, 5, 6, 8, 10, 12, 13, 14, 15, 16, 17, 20, 22])
+def test_real_1d(G, lg):
+    n = 1 << lg
+    x = O.normal(5, n)
+    ref = O.rfft(x.astype(np.float64))
+    h = O.hermitian_input(3, (n,))
+    refi = O.irfft(h.astype(np.complex128), (n,))
+    for fmt in (G.IL, G.SP):
+        say("real 1d", n, "fmt", fmt)
+        z = G.gpu_r2c(x, fmt)
+        assert O.rel_l2(z.astype(np.complex64), ref) <= O.bound(n), (n, fmt)
+        w = G.gpu_c2r(h, (n,), fmt)
+        assert O.rel_l2(w.astype(np.float32), refi) <= O.bound(n), (n, fmt)
+
+
+ND_SHAPES = [(2, 4), (4, 1, 8), (16, 32), (32, 16), (8, 8, 8), (2, 2, 2, 2), (8192, 4), (4, 32768), (64, 64, 64),
+             (2, 8192), (32, 16384)]
+
+
+@pytest.mark.parametrize("shape", ND_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_nd(G, shape):
+    n = int(np.prod(shape))
+    x = O.complex_input(1, shape)
+    xr = O.normal(5, n).reshape(shape)
+    h = O.hermitian_input(3, shape)
+    ref = O.fft(x.astype(np.complex128))
+    refi = O.fft(x.astype(np.complex128), inverse=True)
+    refr = O.rfft(xr.astype(np.float64))
+    refri = O.irfft(h.astype(np.complex128), shape)
+    for fmt in (G.IL, G.SP):
+        say("nd", shape, "fmt", fmt)
+        assert O.rel_l2(G.gpu_c2c(x, False, fmt).astype(np.complex64), ref) <= O.bound(n)
+        assert O.rel_l2(G.gpu_c2c(x, True, fmt).astype(np.complex64), refi) <= O.bound(n)
+        assert O.rel_l2(G.gpu_r2c(xr, fmt).astype(np.complex64), refr) <= O.bound(n)
+        assert O.rel_l2(G.gpu_c2r(h, shape, fmt).astype(np.float32), refri) <= O.bound(n)
+
+
+@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_fp64_path(G, shape):
+    n = int(np.prod(shape))
+    x = O.complex_input(1, shape).astype(np.complex128)
+    xr = O.normal(5, n).reshape(shape).astype(np.float64)
+    h = O.hermitian_input(3, shape).astype(np.complex128)
+    say("fp64", shape)
+    y = G.gpu_c2c(x, False, G.IL, np.float64)
+    assert O.rel_l2(y, O.fft(x)) <= bound64(n)
+    yi = G.gpu_c2c(x, True, G.SP, np.float64)
+    assert O.rel_l2(yi, O.fft(x, inverse=True)) <= bound64(n)
+    z = G.gpu_r2c(xr, G.IL, np.float64)
+    assert O.rel_l2(z, O.rfft(xr)) <= bound64(n)
+    w = G.gpu_c2r(h, shape, G.IL, np.float64)
+    assert O.rel_l2(w, O.irfft(h, shape)) <= bound64(n)
+
+
+def test_roundtrip_scales_by_n(G):
+    for n in (1 << 12, 1 << 20):
+        say("roundtrip", n)
+        x = O.complex_input(4, (n,))
+        y = G.gpu_c2c(G.gpu_c2c(x, False).astype(np.complex64), True)
+        assert O.rel_l2(y.astype(np.complex64), n * x.astype(np.complex128)) <= 2 * O.bound(n)
+        xr = O.normal(6, n)
+        w = G.gpu_c2r(G.gpu_r2c(xr).astype(np.complex64), (n,))
+        assert O.rel_l2(w.astype(np.float32), n * xr.astype(np.float64)) <= 2 * O.bound(n)
+
+
+def test_batched_plan_and_chunk_invariance(G):
+    import torch
+    from fft_amd.afft import Plan
+    n, batch = 1 << 20, 6
+    say("batched plan", n, batch)
+    x = O.complex_input(7, (batch, n))
+    src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+    dst = G.empty_dev(2 * n * batch)
+    p = Plan("c", (n,), batch)
+    p(src, None, dst, None)
+    torch.cuda.synchronize()
+    y = G.to_host(dst)
+    y = (y[0::2] + 1j * y[1::2]).reshape(batch, n)
+    for b in (0, batch - 1):
+        assert O.rel_l2(y[b].astype(np.complex64), O.fft(x[b].astype(np.complex128))) <= O.bound(n)
+    for chunk in (1, 4):
+        p.set_chunk(chunk)
+        dst2 = G.empty_dev(2 * n * batch)
+        p(src, None, dst2, None)
+        torch.cuda.synchronize()
+        assert torch.equal(dst, dst2), chunk
+    p.close()
+
+
+def test_headline_64x2p20_against_device_fp64(G):
+    """The benchmark shape at full size: every transform of the batch against
+    the device FP64 plan (itself pinned to the oracle above), with the FP64
+    error reduction on the device; plus the forward/inverse round trip."""
+    import torch
+    from fft_amd.afft import Plan
+    n, batch = 1 << 20, 64
+    say("headline 64 x 2^20")
+    g = torch.Generator(device="cuda").manual_seed(1234)
+    x32 = torch.rand(batch * n * 2, device="cuda", generator=g, dtype=torch.float32) * 2 - 1
+    x64 = x32.double()
+    y32 = torch.empty_like(x32)
+    y64 = torch.empty_like(x64)
+    Plan("c", (n,), batch)(x32, None, y32, None)
+    Plan("c", (n,), batch, dtype=np.float64)(x64, None, y64, None)
+    torch.cuda.synchronize()
+    err = G.device_rel_l2(y32, y64)
+    assert err <= O.bound(n), err
+    back = torch.empty_like(x32)
+    Plan("ci", (n,), batch)(y32, None, back, None)
+    torch.cuda.synchronize()
+    err_rt = G.device_rel_l2(back, x64 * n)
+    assert err_rt <= 2 * O.bound(n), err_rt
+
+
+def test_large_nd_against_afft_summaries(G):
+    """256^3 c2c and 8192^2 r2c at full size: afft's sampled output (golden)
+    and the device FP64 plan."""
+    import torch
+    from fft_amd.afft import Plan
+    g = np.load(os.path.join(GOLD, "golden_large.npz"), allow_pickle=False)
+    for kind, shape in (("c", (256, 256, 256)), ("r", (8192, 8192)), ("c", (1 << 16,)), ("r", (1 << 22,))):
+        say("large summary", kind, shape)
+        n = int(np.prod(shape))
+        k = f"{kind}|{'x'.join(map(str, shape))}|il|avx2"
+        if kind == "c":
+            x = O.complex_input(1, shape)
+            src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+            m = n
+        else:
+            src = G.to_dev(O.normal(5, n))
+            m = int(np.prod(G.half(shape)))
+        dst = G.empty_dev(2 * m)
+        dst64 = G.empty_dev(2 * m, np.float64)
+        Plan(kind, shape)(src, None, dst, None)
+        Plan(kind, shape, dtype=np.float64)(src.double(), None, dst64, None)
+        torch.cuda.synchronize()
+        assert G.device_rel_l2(dst, dst64) <= O.bound(n)
+        y = G.to_host(dst)
+        yc = y[0::2] + 1j * y[1::2]
+        idx = g[k + "|idx"]
+        gold = g[k + "|samples"].astype(np.complex128)
+        scale = np.sqrt(g[k + "|checksum"][2] / m)  # rms of afft's output
+        assert np.abs(yc[idx] - gold).max() <= 8 * O.bound(n) * scale * np.sqrt(np.log2(n)), k
+
+
+def test_errors_leave_state_clean(G):
+    from fft_amd import _lib
+    import ctypes
+    L = _lib.lib()
+    nd, dims = _lib.dims_array([24])
+    mem = ctypes.create_string_buffer(64)
+    assert not L.afft_32_c_create(nd, dims, 1, mem, 0)
+    assert "power of two" in _lib.last_error()
