@@ -31,6 +31,8 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_plan_scratch_bytes", "afft_gpu_plan_describe", "afft_gpu_last_error",
     "afft_gpu_error_sums_f32", "afft_gpu_error_sums_f64", "afft_gpu_copy", "afft_gpu_execute_range",
     "afft_gpu_plan_launches", "afft_gpu_plan_inspect",
+    "afft_gpu_queue_memsize", "afft_gpu_queue_create", "afft_gpu_queue_prepare", "afft_gpu_queue_launch",
+    "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy",
 ]
 
 
@@ -94,17 +96,20 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_error_sums_f64.restype = ctypes.c_int
     L.afft_gpu_copy.argtypes = [vp, vp, ctypes.c_size_t, vp]
     L.afft_gpu_copy.restype = ctypes.c_int
-    if hasattr(L, "afft_gpu_queue_create"):
-        L.afft_gpu_queue_memsize.argtypes = []
-        L.afft_gpu_queue_memsize.restype = ctypes.c_size_t
-        L.afft_gpu_queue_create.argtypes = [ctypes.c_size_t, ctypes.c_int, vp]
-        L.afft_gpu_queue_create.restype = vp
-        L.afft_gpu_queue_run.argtypes = [vp, vp, ctypes.c_size_t, vp]
-        L.afft_gpu_queue_run.restype = ctypes.c_int
-        L.afft_gpu_queue_run_device.argtypes = [vp, vp, ctypes.c_size_t, vp]
-        L.afft_gpu_queue_run_device.restype = ctypes.c_int
-        L.afft_gpu_queue_destroy.argtypes = [vp]
-        L.afft_gpu_queue_destroy.restype = None
+    L.afft_gpu_queue_memsize.argtypes = []
+    L.afft_gpu_queue_memsize.restype = ctypes.c_size_t
+    L.afft_gpu_queue_create.argtypes = [ctypes.c_size_t, ctypes.c_int, vp]
+    L.afft_gpu_queue_create.restype = vp
+    L.afft_gpu_queue_prepare.argtypes = [vp, vp, ctypes.c_size_t]
+    L.afft_gpu_queue_prepare.restype = ctypes.c_int
+    L.afft_gpu_queue_launch.argtypes = [vp, vp]
+    L.afft_gpu_queue_launch.restype = ctypes.c_int
+    L.afft_gpu_queue_run.argtypes = [vp, vp, ctypes.c_size_t, vp]
+    L.afft_gpu_queue_run.restype = ctypes.c_int
+    L.afft_gpu_queue_num_tasks.argtypes = [vp]
+    L.afft_gpu_queue_num_tasks.restype = ctypes.c_int
+    L.afft_gpu_queue_destroy.argtypes = [vp]
+    L.afft_gpu_queue_destroy.restype = None
     if hasattr(L, "afft_gpu_plan_set_mode"):
         L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
         L.afft_gpu_plan_set_mode.restype = ctypes.c_int

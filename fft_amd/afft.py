@@ -189,3 +189,58 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Job(ctypes.Structure):
+    """afft_gpu_job (include/afft_gpu.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("log2n", ctypes.c_int32), ("src", ctypes.c_uint64),
+                ("dst", ctypes.c_uint64)]
+
+
+class Queue:
+    """Uneven batch of 1-D FP32 transforms run in one persistent launch
+    (afft_gpu_queue_*)."""
+
+    def __init__(self, max_jobs: int, max_log2n: int = 16):
+        L = _lib.lib()
+        self._mem = ctypes.create_string_buffer(L.afft_gpu_queue_memsize())
+        q = L.afft_gpu_queue_create(int(max_jobs), int(max_log2n), self._mem)
+        if not q:
+            raise RuntimeError(f"queue creation failed: {_lib.last_error()}")
+        self.q = q
+
+    @staticmethod
+    def jobs(records):
+        """records: iterable of (kind, log2n, src_addr, dst_addr)."""
+        recs = list(records)
+        arr = (Job * max(1, len(recs)))()
+        for i, (k, lg, s, d) in enumerate(recs):
+            arr[i] = Job(int(k), int(lg), int(s), int(d))
+        return arr, len(recs)
+
+    def prepare(self, jobs) -> int:
+        arr, n = jobs
+        r = _lib.lib().afft_gpu_queue_prepare(self.q, arr, n)
+        if r < 0:
+            raise RuntimeError(_lib.last_error())
+        return r
+
+    def launch(self, stream=None):
+        h = getattr(stream, "cuda_stream", stream)
+        if _lib.lib().afft_gpu_queue_launch(self.q, h) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def run(self, jobs, stream=None):
+        self.prepare(jobs)
+        self.launch(stream)
+
+    def close(self):
+        if self.q:
+            _lib.lib().afft_gpu_queue_destroy(self.q)
+        self.q = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

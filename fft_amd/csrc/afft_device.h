@@ -23,6 +23,10 @@ namespace afgpu {
 
 typedef long long idx_t;
 
+// Every compile-time-loop body must inline, or its register arrays end up in
+// scratch memory when many tiles share one kernel.
+#define AFGPU_INLINE __attribute__((always_inline))
+
 template <typename T>
 struct cx {
   T x, y;
@@ -134,25 +138,25 @@ struct Dft {
     } else {
       constexpr int P = R >= 16 ? 4 : 2;
       constexpr int Q = R / P;
-      static_for<Q>([&](auto n2c) {
+      static_for<Q>([&](auto n2c) AFGPU_INLINE {
         constexpr int n2 = decltype(n2c)::value;
         cx<T> a[P];
-        static_for<P>([&](auto n1c) { a[decltype(n1c)::value] = v[Q * decltype(n1c)::value + n2]; });
+        static_for<P>([&](auto n1c) AFGPU_INLINE { a[decltype(n1c)::value] = v[Q * decltype(n1c)::value + n2]; });
         Dft<T, P>::run(a);
-        static_for<P>([&](auto k1c) {
+        static_for<P>([&](auto k1c) AFGPU_INLINE {
           constexpr int k1 = decltype(k1c)::value;
           v[Q * k1 + n2] = twc<T, R, n2 * k1>(a[k1]);
         });
       });
       cx<T> out[R];
-      static_for<P>([&](auto k1c) {
+      static_for<P>([&](auto k1c) AFGPU_INLINE {
         constexpr int k1 = decltype(k1c)::value;
         cx<T> b[Q];
-        static_for<Q>([&](auto n2c) { b[decltype(n2c)::value] = v[Q * k1 + decltype(n2c)::value]; });
+        static_for<Q>([&](auto n2c) AFGPU_INLINE { b[decltype(n2c)::value] = v[Q * k1 + decltype(n2c)::value]; });
         Dft<T, Q>::run(b);
-        static_for<Q>([&](auto k2c) { out[k1 + P * decltype(k2c)::value] = b[decltype(k2c)::value]; });
+        static_for<Q>([&](auto k2c) AFGPU_INLINE { out[k1 + P * decltype(k2c)::value] = b[decltype(k2c)::value]; });
       });
-      static_for<R>([&](auto i) { v[decltype(i)::value] = out[decltype(i)::value]; });
+      static_for<R>([&](auto i) AFGPU_INLINE { v[decltype(i)::value] = out[decltype(i)::value]; });
     }
   }
 };

@@ -66,6 +66,10 @@ struct PassParams {
   int tw4_lo_bits;
   int tw4_col_shift;      // n2 = column >> shift
   const cx<T>* real_tw;   // W_{2L}^k, k < L
+  // Optional per-column buffers (uneven batches): column c reads from
+  // col_ptrs[2c] and writes to col_ptrs[2c+1], both interleaved, with the
+  // batch and column offsets taken as zero.
+  const unsigned long long* col_ptrs;
 };
 
 template <typename T>
@@ -101,34 +105,34 @@ __device__ __forceinline__ void run_stages(cx<T> (&v)[Sched<L>::E], int t, LdsAt
   constexpr int NS = StageNs<L, S>::value;
   constexpr int NB = E / R;
   constexpr bool LAST = (S == Sched<L>::NST - 1);
-  static_for<NB>([&](auto bc) {
+  static_for<NB>([&](auto bc) AFGPU_INLINE {
     constexpr int b = decltype(bc)::value;
     cx<T> a[R];
-    static_for<R>([&](auto rc) { a[decltype(rc)::value] = v[b + decltype(rc)::value * NB]; });
+    static_for<R>([&](auto rc) AFGPU_INLINE { a[decltype(rc)::value] = v[b + decltype(rc)::value * NB]; });
     if constexpr (NS > 1) {
       const int j = t + b * TPT;
       const int k = j & (NS - 1);
       const cx<T>* tp = tw + StageTwOff<L, S>::value + k * (R - 1);
-      static_for<R - 1>([&](auto rc) {
+      static_for<R - 1>([&](auto rc) AFGPU_INLINE {
         constexpr int r = decltype(rc)::value + 1;
         a[r] = cmul(a[r], tp[r - 1]);
       });
     }
     Dft<T, R>::run(a);
-    static_for<R>([&](auto rc) { v[b + decltype(rc)::value * NB] = a[decltype(rc)::value]; });
+    static_for<R>([&](auto rc) AFGPU_INLINE { v[b + decltype(rc)::value * NB] = a[decltype(rc)::value]; });
   });
   if constexpr (!LAST) {
-    static_for<NB>([&](auto bc) {
+    static_for<NB>([&](auto bc) AFGPU_INLINE {
       constexpr int b = decltype(bc)::value;
       const int j = t + b * TPT;
       const int base = (j / NS) * (NS * R) + (j & (NS - 1));
-      static_for<R>([&](auto rc) {
+      static_for<R>([&](auto rc) AFGPU_INLINE {
         constexpr int r = decltype(rc)::value;
         lat(base + r * NS) = v[b + r * NB];
       });
     });
     __syncthreads();
-    static_for<E>([&](auto mc) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
       constexpr int m = decltype(mc)::value;
       v[m] = lat(t + m * TPT);
     });
@@ -146,19 +150,17 @@ struct PassShape {
   static constexpr size_t lds_elems = size_t(CT) * LL;
 };
 
+// One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
+// tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
-__global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
+__device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
   using PS = PassShape<L, CT>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
   constexpr int LL = PS::LL;
-  extern __shared__ __align__(16) unsigned char afgpu_smem[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(afgpu_smem);
 
-  const int tid = threadIdx.x;
   const int c = COLMAP ? (tid % CT) : (tid / TPT);
   const int t = COLMAP ? (tid / CT) : (tid % TPT);
-  const int tile = blockIdx.x;
   const int bidx = tile / p.col_tiles;
   const int col0 = (tile - bidx * p.col_tiles) * CT;
   const int col = col0 + c;
@@ -173,15 +175,29 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
       return lds[c * LL + lds_pad(i)];
   };
 
+  const T* in_re = p.in_re;
+  const T* in_im = p.in_im;
+  T* out_re = p.out_re;
+  T* out_im = p.out_im;
+  idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + (idx_t)col * p.in_cs;
+  idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)col * p.out_cs;
+  if (p.col_ptrs) {
+    ib = 0;
+    ob = 0;
+    if (active) {
+      in_re = reinterpret_cast<const T*>(p.col_ptrs[2 * col]);
+      out_re = reinterpret_cast<T*>(p.col_ptrs[2 * col + 1]);
+    }
+  }
+
   cx<T> v[E];
-  const idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + (idx_t)col * p.in_cs;
   if (active) {
     if constexpr (EPI == EPI_C2R) {
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const int k = t + m * TPT;
-        cx<T> s0 = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)k * p.in_js);
-        cx<T> s1 = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js);
+        cx<T> s0 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)k * p.in_js);
+        cx<T> s1 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js);
         cx<T> z;
         if (k == 0) {
           z = {s0.x + s1.x, s0.x - s1.x};
@@ -195,27 +211,26 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
         v[m] = p.swap_in ? swapri(z) : z;
       });
     } else {
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
-        cx<T> z = ld_elem(p.in_re, p.in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js);
+        cx<T> z = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js);
         v[m] = p.swap_in ? swapri(z) : z;
       });
     }
   } else {
-    static_for<E>([&](auto mc) { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
+    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
   }
 
   run_stages<T, L, 0>(v, t, lat, p.stage_tw);
 
-  const idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)col * p.out_cs;
   if constexpr (EPI == EPI_R2C) {
-    static_for<E>([&](auto mc) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
       constexpr int m = decltype(mc)::value;
       lat(t + m * TPT) = v[m];
     });
     __syncthreads();
     if (active) {
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const int k = t + m * TPT;
         cx<T> s0 = v[m];
@@ -223,7 +238,7 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
         if (k == 0) {
           x = {s0.x + s0.y, T(0)};
           cx<T> xn = {s0.x - s0.y, T(0)};
-          st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn);
+          st_elem(out_re, out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn);
         } else {
           cx<T> s1c = conjg(lat(L - k));
           cx<T> w = p.real_tw[k];
@@ -234,13 +249,13 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
           cx<T> bb = {bw.x * h, bw.y * h};
           x = a + mul_neg_i(bb);
         }
-        st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x);
+        st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x);
       });
     }
   } else {
     if constexpr (EPI == EPI_TW4) {
       const idx_t n2 = (idx_t)(col >> p.tw4_col_shift);
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const idx_t k1 = t + m * TPT;
         v[m] = cmul(v[m], tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, n2 * k1));
@@ -248,7 +263,7 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
     }
     if constexpr (STORE_T) {
       static_assert(!COLMAP, "transposed store is for row-mapped passes");
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         lat(t + m * TPT) = v[m];
       });
@@ -258,7 +273,7 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
       const int colb = col0 + c2;
       if (colb < p.n_col) {
         const idx_t ob2 = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
-        static_for<E>([&](auto mc) {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
           const int k = t2 + m * TPT;
           cx<T> z = lds[c2 * LL + lds_pad(k)];
@@ -266,13 +281,19 @@ __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassPa
         });
       }
     } else if (active) {
-      static_for<E>([&](auto mc) {
+      static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         cx<T> z = v[m];
-        st_elem(p.out_re, p.out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z);
+        st_elem(out_re, out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z);
       });
     }
   }
+}
+
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
+__global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  fft_tile<T, L, CT, COLMAP, EPI, STORE_T>(p, blockIdx.x, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
 }  // namespace afgpu

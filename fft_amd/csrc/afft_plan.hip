@@ -879,6 +879,10 @@ void transform_impl(void* state, const void* sre, const void* sim, void* dre, vo
 
 // ---- stage schedule shared with the twiddle generator --------------------------
 namespace afgpu {
+// entry points for afft_queue.hip
+const void* queue_table(int type, idx_t n, int bits) { return get_table<float>(type, n, bits); }
+void queue_set_err(const char* msg) { set_err("%s", msg); }
+
 template <int L>
 static int fill_stages(int* radix, int* ns) {
   int acc = 1;

@@ -78,6 +78,37 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
 /* The last error message of the calling thread ("" if none). */
 const char* afft_gpu_last_error(void);
 
+/* ---- uneven batches in one persistent launch ------------------------------ */
+/* One job of a mixed batch: a 1-D FP32 transform of n = 2^log2n points,
+ * interleaved complex, with afft's semantics for the kind:
+ *   afft_gpu_c / afft_gpu_ci : n complex in, n complex out;
+ *   afft_gpu_r               : n reals in, n/2+1 complex out;
+ *   afft_gpu_ri              : n/2+1 complex in, n reals out. */
+typedef struct
+{
+  int32_t kind;
+  int32_t log2n;
+  uint64_t src; /* device address of the input */
+  uint64_t dst; /* device address of the output (distinct from src) */
+} afft_gpu_job;
+
+typedef struct afft_gpu_queue_struct afft_gpu_queue;
+
+/* Bytes of host memory a queue header needs. */
+size_t afft_gpu_queue_memsize(void);
+/* A queue for up to max_jobs jobs of sizes up to 2^max_log2n (<= 2^16). */
+afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem);
+/* Groups the jobs into tasks and uploads the task list (synchronous host
+ * work).  Returns the number of tasks, or -1. */
+int afft_gpu_queue_prepare(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs);
+/* Runs the prepared task list in one persistent launch on `stream`.  Issues
+ * kernels only, so it can be captured in a HIP graph and replayed. */
+int afft_gpu_queue_launch(afft_gpu_queue* q, void* stream);
+/* prepare + launch. */
+int afft_gpu_queue_run(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs, void* stream);
+int afft_gpu_queue_num_tasks(const afft_gpu_queue* q);
+void afft_gpu_queue_destroy(afft_gpu_queue* q);
+
 /* ---- plan inspection (host only, no GPU needed) ---------------------------- */
 /* Buffer side of a pass: role 0 = src, 1 = dst, 2/3 = plan scratch.  il = 1:
  * interleaved complex addressing (offsets/strides in complex elements); il = 0:
