@@ -44,6 +44,14 @@ def lib() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is not built; run `python -m fft_amd.build` (hipcc, gfx950) first")
+    # PyTorch ships its own libamdhip64.so.7.  Loading torch first makes this
+    # library bind to that same HIP runtime (matched by soname), so device
+    # pointers, streams and graph capture are shared; loading this library
+    # first would bring in a second runtime and hide the GPU from torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for p, T in (("32", ctypes.c_float), ("64", ctypes.c_double)):
         for k in ("c", "ci", "r", "ri"):

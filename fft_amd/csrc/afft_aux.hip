@@ -22,7 +22,8 @@ template <typename T>
 __global__ void gen_stage_tw_kernel(cx<T>* out, StageList sl) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= sl.total) return;
-  for (int s = 0; s < sl.nstage; s++) {
+  // stage 0 (Ns = 1) multiplies by nothing and has no entries
+  for (int s = 1; s < sl.nstage; s++) {
     int beg = sl.off[s], cnt = sl.ns[s] * (sl.radix[s] - 1);
     if (i >= beg && i < beg + cnt) {
       int rel = i - beg;
@@ -86,13 +87,16 @@ hipError_t gen_real_tw(cx<T>* out, idx_t L, hipStream_t st) {
 // Real pass over rows of m complex entries (h + 1 rows per batch entry): row k
 // is paired with row h - k.  Forward (r2c post-pass) and inverse (c2r
 // pre-pass) follow onedim.hpp:1031-1108 (m = 1) and multi.hpp:343-414; src and
-// dst may alias.  grid = (ceil(m / 256), h/2 + 1, nbatch).
+// dst may alias.  One thread per (batch, pair k, column c), c fastest.
 template <typename T, bool INV>
 __global__ void real_rows_kernel(RealRowsArgs<T> a) {
-  const idx_t c = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const idx_t k = blockIdx.y;
-  const idx_t b = blockIdx.z;
-  if (c >= a.m) return;
+  const idx_t npair = a.h / 2 + 1;
+  const idx_t total = a.nbatch * npair * a.m;
+  const idx_t i = (idx_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const idx_t c = i % a.m;
+  const idx_t k = (i / a.m) % npair;
+  const idx_t b = i / (a.m * npair);
   const idx_t ib = b * a.in_bs + c * a.in_cs;
   const idx_t ob = b * a.out_bs + c * a.out_cs;
   auto ld = [&](idx_t row) -> cx<T> {
@@ -145,11 +149,13 @@ __global__ void real_rows_kernel(RealRowsArgs<T> a) {
 template <typename T>
 hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st) {
   if (a.h <= 0 || a.m <= 0 || a.nbatch <= 0) return hipSuccess;
-  dim3 grid((unsigned)((a.m + 255) / 256), (unsigned)(a.h / 2 + 1), (unsigned)a.nbatch);
+  const idx_t total = a.nbatch * (a.h / 2 + 1) * a.m;
+  const idx_t blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   if (inverse)
-    hipLaunchKernelGGL((real_rows_kernel<T, true>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((real_rows_kernel<T, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL((real_rows_kernel<T, false>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((real_rows_kernel<T, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
