@@ -32,6 +32,19 @@ struct cx {
   T x, y;
 };
 
+// Two-element vector types, for 8 / 16-byte accesses through the
+// non-temporal builtins (which take scalar or vector element types only).
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Vec2<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
 template <typename T>
 __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

@@ -60,6 +60,8 @@ struct PassParams {
   int n_tiles;            // col_tiles * n_batch
   int in_il, out_il;
   int swap_in, swap_out;
+  int in_nt, out_nt;  // streaming (non-temporal) loads / stores
+  int xcd_remap;      // give each XCD a contiguous run of tiles
   const cx<T>* stage_tw;  // per-length stage twiddles
   const cx<T>* tw4_hi;    // W_N^(i << lo_bits)
   const cx<T>* tw4_lo;    // W_N^j - 1, j < 2^lo_bits
@@ -73,18 +75,45 @@ struct PassParams {
 };
 
 template <typename T>
-__device__ __forceinline__ cx<T> ld_elem(const T* __restrict__ re, const T* __restrict__ im, int il, idx_t e) {
+__device__ __forceinline__ cx<T> ld_elem(const T* __restrict__ re, const T* __restrict__ im, int il, idx_t e,
+                                         int nt = 0) {
+  if (nt) {
+    if (il) {
+      typename Vec2<T>::type w = __builtin_nontemporal_load(reinterpret_cast<const typename Vec2<T>::type*>(re) + e);
+      return {w.x, w.y};
+    }
+    return {__builtin_nontemporal_load(re + e), __builtin_nontemporal_load(im + e)};
+  }
   if (il) return reinterpret_cast<const cx<T>*>(re)[e];
   return {re[e], im[e]};
 }
 template <typename T>
-__device__ __forceinline__ void st_elem(T* __restrict__ re, T* __restrict__ im, int il, idx_t e, cx<T> v) {
+__device__ __forceinline__ void st_elem(T* __restrict__ re, T* __restrict__ im, int il, idx_t e, cx<T> v,
+                                        int nt = 0) {
+  if (nt) {
+    if (il) {
+      typename Vec2<T>::type w = {v.x, v.y};
+      __builtin_nontemporal_store(w, reinterpret_cast<typename Vec2<T>::type*>(re) + e);
+    } else {
+      __builtin_nontemporal_store(v.x, re + e);
+      __builtin_nontemporal_store(v.y, im + e);
+    }
+    return;
+  }
   if (il) {
     reinterpret_cast<cx<T>*>(re)[e] = v;
   } else {
     re[e] = v.x;
     im[e] = v.y;
   }
+}
+
+// Work-groups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Remapping block b to tile (b % 8) * (n / 8) + b / 8 hands each XCD a
+// contiguous run of tiles, so tiles that share cache lines meet in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  if ((n & 7) != 0) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
 }
 
 // W_N^m from the two-level table: hi[m >> b] * (1 + lo[m & mask]).
@@ -196,8 +225,8 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const int k = t + m * TPT;
-        cx<T> s0 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)k * p.in_js);
-        cx<T> s1 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js);
+        cx<T> s0 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)k * p.in_js, p.in_nt);
+        cx<T> s1 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js, p.in_nt);
         cx<T> z;
         if (k == 0) {
           z = {s0.x + s1.x, s0.x - s1.x};
@@ -213,7 +242,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
     } else {
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
-        cx<T> z = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js);
+        cx<T> z = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js, p.in_nt);
         v[m] = p.swap_in ? swapri(z) : z;
       });
     }
@@ -238,7 +267,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
         if (k == 0) {
           x = {s0.x + s0.y, T(0)};
           cx<T> xn = {s0.x - s0.y, T(0)};
-          st_elem(out_re, out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn);
+          st_elem(out_re, out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn, p.out_nt);
         } else {
           cx<T> s1c = conjg(lat(L - k));
           cx<T> w = p.real_tw[k];
@@ -249,7 +278,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           cx<T> bb = {bw.x * h, bw.y * h};
           x = a + mul_neg_i(bb);
         }
-        st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x);
+        st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x, p.out_nt);
       });
     }
   } else {
@@ -277,14 +306,14 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           constexpr int m = decltype(mc)::value;
           const int k = t2 + m * TPT;
           cx<T> z = lds[c2 * LL + lds_pad(k)];
-          st_elem(p.out_re, p.out_im, p.out_il, ob2 + (idx_t)k * p.out_ks, p.swap_out ? swapri(z) : z);
+          st_elem(p.out_re, p.out_im, p.out_il, ob2 + (idx_t)k * p.out_ks, p.swap_out ? swapri(z) : z, p.out_nt);
         });
       }
     } else if (active) {
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         cx<T> z = v[m];
-        st_elem(out_re, out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z);
+        st_elem(out_re, out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z, p.out_nt);
       });
     }
   }
@@ -293,7 +322,8 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
 __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
-  fft_tile<T, L, CT, COLMAP, EPI, STORE_T>(p, blockIdx.x, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
+  const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
+  fft_tile<T, L, CT, COLMAP, EPI, STORE_T>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
 }  // namespace afgpu

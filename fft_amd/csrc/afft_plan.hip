@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -94,9 +95,19 @@ struct PassRec {
   const void* real_tw = nullptr;
   idx_t rr_h = 0, rr_m = 0, rr_rs = 0;
   int rr_inv = 0;
+  int in_nt = 0, out_nt = 0, xcd = 0;
+};
+
+struct Variant {
+  int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
+  int rowt_tile = 0;  // rows per group of transposed-store passes (0 = default)
+  int stream_io = 0;  // non-temporal loads of src / stores of dst
+  int xcd_order = 0;  // XCD-contiguous tile order
+  idx_t chunk = 0;    // transforms per pass group (0 = default)
 };
 
 struct Plan {
+  Variant var;
   int prec = 32;
   int kind = 0;
   int format = 1;
@@ -224,7 +235,11 @@ struct Builder {
 
   bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t) {
     int CT = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
-    const PassVariant<T>* v = find_pass<T>(L, CT, colmap, epi, store_t);
+    const PassVariant<T>* v = nullptr;
+    if (colmap && P.var.col_tile > 0) v = find_pass<T>(L, P.var.col_tile, colmap, epi, store_t);
+    if (store_t && P.var.rowt_tile > 0) v = find_pass<T>(L, P.var.rowt_tile, colmap, epi, store_t);
+    if (v) CT = v->CT;
+    if (!v) v = find_pass<T>(L, CT, colmap, epi, store_t);
     if (!v) {
       set_err("no kernel for L=%d colmap=%d epi=%d store_t=%d", L, colmap, epi, store_t);
       ok = false;
@@ -442,6 +457,19 @@ struct Builder {
   }
 
   bool build(int kind, int format, const idx_t* d, int nd) {
+    P.npass = 0;
+    P.scr_per_user[0] = P.scr_per_user[1] = 0;
+    bool r = build_passes(kind, format, d, nd);
+    for (int i = 0; i < P.npass; i++) {
+      PassRec& q = P.pass[i];
+      q.in_nt = P.var.stream_io && q.in.role == R_SRC;
+      q.out_nt = P.var.stream_io && q.out.role == R_DST;
+      q.xcd = P.var.xcd_order;
+    }
+    return r && ok;
+  }
+
+  bool build_passes(int kind, int format, const idx_t* d, int nd) {
     const int swap = (kind == 1) ? 1 : 0;
     idx_t total = 1;
     for (int i = 0; i < nd; i++) total *= d[i];
@@ -676,6 +704,9 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     p.out_il = r.out.il;
     p.swap_in = r.swap_in;
     p.swap_out = r.swap_out;
+    p.in_nt = r.in_nt;
+    p.out_nt = r.out_nt;
+    p.xcd_remap = r.xcd;
     p.stage_tw = (const cx<T>*)r.stage_tw;
     p.tw4_hi = (const cx<T>*)r.tw4_hi;
     p.tw4_lo = (const cx<T>*)r.tw4_lo;
@@ -756,6 +787,7 @@ bool alloc_scratch(Plan& P, size_t elem_bytes) {
 idx_t default_chunk(const Plan& P, size_t elem_bytes) {
   idx_t per = P.scr_per_user[0] > P.scr_per_user[1] ? P.scr_per_user[0] : P.scr_per_user[1];
   if (per == 0) return P.batch;
+  if (P.var.chunk > 0) return P.var.chunk < P.batch ? P.var.chunk : P.batch;
   // keep one pass group's scratch within ~64 MiB so it stays in the MALL
   idx_t target = (idx_t)(64u << 20) / (idx_t)(per * elem_bytes);
   if (target < 1) target = 1;
@@ -793,6 +825,8 @@ int validate(size_t ndim, const size_t* dim, int format, int impl, idx_t* d, int
   return 0;
 }
 
+bool tcache_get(const Plan& P, Variant& v);  // defined with the autotuner below
+
 template <typename T>
 void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t batch, void* mem, int impl) {
   g_err.clear();
@@ -824,6 +858,8 @@ void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t b
   for (int i = 0; i < nd; i++) P->dims[i] = d[i];
   P->batch = (idx_t)batch;
   (void)hipGetDevice(&P->device);
+  Variant cached;
+  if (tcache_get(*P, cached)) P->var = cached;
   Builder<T> b(*P);
   if (!b.build(kind, format, d, nd)) {
     delete P;
@@ -873,6 +909,132 @@ void transform_impl(void* state, const void* sre, const void* sim, void* dre, vo
   if (!execute<T>(P, sre, sim, dre, dim, P.stream)) return;
   hipError_t e = hipStreamSynchronize(P.stream);
   if (e != hipSuccess) set_err("transform: %s", hipGetErrorString(e));
+}
+
+
+// ---- variants and the autotuner ---------------------------------------------------
+std::string plan_key(const Plan& P) {
+  std::string k = std::to_string(P.prec) + "/" + std::to_string(P.kind) + "/" + std::to_string(P.format) + "/" +
+                  std::to_string((long long)P.batch) + "/";
+  for (int i = 0; i < P.ndim; i++) k += (i ? "x" : "") + std::to_string((long long)P.dims[i]);
+  return k;
+}
+
+struct TuneCache {
+  std::mutex mu;
+  bool loaded = false;
+  std::map<std::string, Variant> best;
+};
+TuneCache& tcache() {
+  static TuneCache* c = new TuneCache;
+  return *c;
+}
+
+// The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk>
+void tcache_load_locked(TuneCache& c) {
+  if (c.loaded) return;
+  c.loaded = true;
+  const char* path = getenv("AFFT_GPU_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  char key[512];
+  Variant v;
+  long long ch = 0;
+  while (fscanf(f, "%511s %d %d %d %d %lld", key, &v.col_tile, &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch) == 6) {
+    v.chunk = ch;
+    c.best[key] = v;
+  }
+  fclose(f);
+}
+
+bool tcache_get(const Plan& P, Variant& v) {
+  TuneCache& c = tcache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  tcache_load_locked(c);
+  auto it = c.best.find(plan_key(P));
+  if (it == c.best.end()) return false;
+  v = it->second;
+  return true;
+}
+
+void tcache_put(const Plan& P, const Variant& v) {
+  TuneCache& c = tcache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  tcache_load_locked(c);
+  c.best[plan_key(P)] = v;
+  const char* path = getenv("AFFT_GPU_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %d %d %lld\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
+          (long long)v.chunk);
+  fclose(f);
+}
+
+template <typename T>
+bool rebuild(Plan& P) {
+  Builder<T> b(P);
+  if (!b.build(P.kind, P.format, P.dims, P.ndim)) return false;
+  P.chunk = default_chunk(P, sizeof(cx<T>));
+  return alloc_scratch(P, sizeof(cx<T>));
+}
+
+bool rebuild_any(Plan& P) { return P.prec == 32 ? rebuild<float>(P) : rebuild<double>(P); }
+
+template <typename T>
+bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int iters,
+               float* best_ms) {
+  const Variant keep = P.var;
+  std::vector<Variant> cands;
+  idx_t base_chunk = 0;
+  {
+    P.var = Variant();
+    if (!rebuild<T>(P)) return false;
+    base_chunk = P.chunk;
+  }
+  for (int col : {0, 8})
+    for (int rowt : {0, 8})
+      for (int sio : {0, 1})
+        for (int xcd : {0, 1})
+          for (int cm : {1, 2, 4}) {
+            Variant v;
+            v.col_tile = col;
+            v.rowt_tile = rowt;
+            v.stream_io = sio;
+            v.xcd_order = xcd;
+            v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
+            if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
+            cands.push_back(v);
+          }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
+  float best = 1e30f;
+  Variant bestv = keep;
+  for (const Variant& v : cands) {
+    P.var = v;
+    if (!rebuild<T>(P)) continue;
+    if (!execute<T>(P, sre, sim, dre, dim, st)) continue;
+    (void)hipEventRecord(e0, st);
+    for (int i = 0; i < iters; i++) execute<T>(P, sre, sim, dre, dim, st);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= iters;
+    if (ms < best) {
+      best = ms;
+      bestv = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P.var = bestv;
+  if (!rebuild<T>(P)) return false;
+  if (best_ms) *best_ms = best;
+  tcache_put(P, bestv);
+  return best < 1e29f;
 }
 
 }  // namespace
@@ -1043,6 +1205,49 @@ int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk) {
     return -1;
   }
   return 0;
+}
+
+int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
+  g_err.clear();
+  Header h;
+  if (!read_header(plan, h) || !v) return -1;
+  Plan& P = *h.plan;
+  Variant old = P.var;
+  P.var.col_tile = v->col_tile;
+  P.var.rowt_tile = v->rowt_tile;
+  P.var.stream_io = v->stream_io;
+  P.var.xcd_order = v->xcd_order;
+  P.var.chunk = v->chunk;
+  if (!rebuild_any(P)) {
+    P.var = old;
+    rebuild_any(P);
+    return -1;
+  }
+  return 0;
+}
+
+int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
+  Header h;
+  if (!read_header(plan, h) || !v) return -1;
+  const Plan& P = *h.plan;
+  v->col_tile = P.var.col_tile;
+  v->rowt_tile = P.var.rowt_tile;
+  v->stream_io = P.var.stream_io;
+  v->xcd_order = P.var.xcd_order;
+  v->chunk = (int64_t)P.chunk;
+  return 0;
+}
+
+int afft_gpu_plan_tune(afft_gpu_plan* plan, const void* sre, const void* sim, void* dre, void* dim, void* stream,
+                       int iters, float* best_ms) {
+  g_err.clear();
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  Plan& P = *h.plan;
+  if (iters < 1) iters = 3;
+  bool ok = P.prec == 32 ? tune_impl<float>(P, sre, sim, dre, dim, (hipStream_t)stream, iters, best_ms)
+                         : tune_impl<double>(P, sre, sim, dre, dim, (hipStream_t)stream, iters, best_ms);
+  return ok ? 0 : -1;
 }
 
 int afft_gpu_plan_num_passes(const afft_gpu_plan* plan) {

@@ -68,6 +68,29 @@ int afft_gpu_set_stream(void* state, void* stream);
  * (0 = automatic).  Tuning knob; the result does not depend on it. */
 int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk);
 
+/* Kernel-shape variant of a plan.  Every variant computes the same result;
+ * they differ in speed.  0 means "default" for the integer fields. */
+typedef struct
+{
+  int32_t col_tile;  /* columns per work-group of column passes with L >= 512 (8 or 16) */
+  int32_t rowt_tile; /* rows per work-group of transposed-store row passes (8 or 16) */
+  int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
+  int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
+  int64_t chunk;     /* transforms per pass group of four-step plans */
+} afft_gpu_variant;
+
+int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);
+int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v);
+
+/* Autotuner: runs the plan `iters` times per candidate variant on the given
+ * buffers (dst is overwritten; src is read only) on `stream`, keeps the
+ * fastest and records it in the process-wide cache keyed by (precision,
+ * kind, format, batch, dims).  Later plans of the same key start from the
+ * tuned variant; with AFFT_GPU_TUNE_CACHE=<file> the cache persists across
+ * processes.  best_ms (optional) receives the winner's time per execute. */
+int afft_gpu_plan_tune(afft_gpu_plan* plan, const void* src_re, const void* src_im, void* dst_re, void* dst_im,
+                       void* stream, int iters, float* best_ms);
+
 /* Plan introspection: number of kernel launches per execute, and the scratch
  * bytes held on the device. */
 int afft_gpu_plan_num_passes(const afft_gpu_plan* plan);
