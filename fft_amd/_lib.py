@@ -33,6 +33,8 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_plan_launches", "afft_gpu_plan_inspect",
     "afft_gpu_queue_memsize", "afft_gpu_queue_create", "afft_gpu_queue_prepare", "afft_gpu_queue_launch",
     "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy",
+    "afft_gpu_plan_set_mode", "afft_gpu_mfma_selftest", "afft_gpu_plan_set_variant", "afft_gpu_plan_get_variant",
+    "afft_gpu_plan_tune",
 ]
 
 
@@ -118,9 +120,16 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_queue_num_tasks.restype = ctypes.c_int
     L.afft_gpu_queue_destroy.argtypes = [vp]
     L.afft_gpu_queue_destroy.restype = None
-    if hasattr(L, "afft_gpu_plan_set_mode"):
-        L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
-        L.afft_gpu_plan_set_mode.restype = ctypes.c_int
+    L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
+    L.afft_gpu_plan_set_mode.restype = ctypes.c_int
+    L.afft_gpu_mfma_selftest.argtypes = []
+    L.afft_gpu_mfma_selftest.restype = ctypes.c_int
+    L.afft_gpu_plan_set_variant.argtypes = [vp, vp]
+    L.afft_gpu_plan_set_variant.restype = ctypes.c_int
+    L.afft_gpu_plan_get_variant.argtypes = [vp, vp]
+    L.afft_gpu_plan_get_variant.restype = ctypes.c_int
+    L.afft_gpu_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
+    L.afft_gpu_plan_tune.restype = ctypes.c_int
     _LIB = L
     return L
 

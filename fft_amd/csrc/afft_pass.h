@@ -68,6 +68,8 @@ struct PassParams {
   int tw4_lo_bits;
   int tw4_col_shift;      // n2 = column >> shift
   const cx<T>* real_tw;   // W_{2L}^k, k < L
+  // Byte extents of the in / out regions of one batch entry (fast path only).
+  unsigned in_bytes, out_bytes;
   // Optional per-column buffers (uneven batches): column c reads from
   // col_ptrs[2c] and writes to col_ptrs[2c+1], both interleaved, with the
   // batch and column offsets taken as zero.
@@ -181,12 +183,63 @@ struct PassShape {
 
 // One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
 // tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
+// ---- element access ---------------------------------------------------------
+// Generic access follows PassParams in full (split / interleaved, one-array
+// split, per-column buffers).  FAST access is for interleaved data on both
+// sides: one buffer resource per tile side (base and byte extent uniform over
+// the work-group, so reads past the extent return 0 and writes past it are
+// dropped) and 32-bit byte offsets per element, which saves the 64-bit address
+// arithmetic of every load and store.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int AUX>
+__device__ __forceinline__ cx<T> buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ cx<float> buf_ld<float, 0>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return {__uint_as_float(w.x), __uint_as_float(w.y)};
+}
+template <>
+__device__ __forceinline__ cx<float> buf_ld<float, 2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 2);
+  return {__uint_as_float(w.x), __uint_as_float(w.y)};
+}
+template <>
+__device__ __forceinline__ cx<double> buf_ld<double, 0>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return {__hiloint2double(w.y, w.x), __hiloint2double(w.w, w.z)};
+}
+template <>
+__device__ __forceinline__ cx<double> buf_ld<double, 2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 2);
+  return {__hiloint2double(w.y, w.x), __hiloint2double(w.w, w.z)};
+}
+template <int AUX>
+__device__ __forceinline__ void buf_st(cx<float> v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  u32x2 w = {__float_as_uint(v.x), __float_as_uint(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, off, 0, AUX);
+}
+template <int AUX>
+__device__ __forceinline__ void buf_st(cx<double> v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  unsigned long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
+  u32x4 w = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(w, r, off, 0, AUX);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+
+// One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
+// tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
   using PS = PassShape<L, CT>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
   constexpr int LL = PS::LL;
+  constexpr unsigned ES = sizeof(cx<T>);
 
   const int c = COLMAP ? (tid % CT) : (tid / TPT);
   const int t = COLMAP ? (tid / CT) : (tid % TPT);
@@ -204,20 +257,57 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       return lds[c * LL + lds_pad(i)];
   };
 
+  // generic addressing
   const T* in_re = p.in_re;
   const T* in_im = p.in_im;
   T* out_re = p.out_re;
   T* out_im = p.out_im;
-  idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + (idx_t)col * p.in_cs;
-  idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)col * p.out_cs;
-  if (p.col_ptrs) {
-    ib = 0;
-    ob = 0;
-    if (active) {
-      in_re = reinterpret_cast<const T*>(p.col_ptrs[2 * col]);
-      out_re = reinterpret_cast<T*>(p.col_ptrs[2 * col + 1]);
+  idx_t ib = 0, ob = 0;
+  // fast addressing: per-side resource + 32-bit byte offsets
+  __amdgpu_buffer_rsrc_t rin = make_rsrc(p.in_re, 0u), rout = make_rsrc(p.out_re, 0u);
+  unsigned in0 = 0, out0 = 0, js8 = 0, ks8 = 0;
+  if constexpr (FAST) {
+    rin = make_rsrc(reinterpret_cast<const cx<T>*>(p.in_re) + ((idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2),
+                    p.in_bytes);
+    rout = make_rsrc(reinterpret_cast<const cx<T>*>(p.out_re) + ((idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2),
+                     p.out_bytes);
+    in0 = (unsigned)col * (unsigned)p.in_cs * ES;
+    out0 = (unsigned)col * (unsigned)p.out_cs * ES;
+    js8 = (unsigned)p.in_js * ES;
+    ks8 = (unsigned)p.out_ks * ES;
+  } else {
+    ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + (idx_t)col * p.in_cs;
+    ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)col * p.out_cs;
+    if (p.col_ptrs) {
+      ib = 0;
+      ob = 0;
+      if (active) {
+        in_re = reinterpret_cast<const T*>(p.col_ptrs[2 * col]);
+        out_re = reinterpret_cast<T*>(p.col_ptrs[2 * col + 1]);
+      }
     }
   }
+  // element j of this column / bin k of this column
+  auto load = [&](int j) AFGPU_INLINE -> cx<T> {
+    if constexpr (FAST) {
+      const unsigned off = in0 + (unsigned)j * js8;
+      return p.in_nt ? buf_ld<T, 2>(rin, off) : buf_ld<T, 0>(rin, off);
+    } else {
+      return ld_elem(in_re, in_im, p.in_il, ib + (idx_t)j * p.in_js, p.in_nt);
+    }
+  };
+  auto store = [&](int k, cx<T> z) AFGPU_INLINE {
+    if (p.swap_out) z = swapri(z);
+    if constexpr (FAST) {
+      const unsigned off = out0 + (unsigned)k * ks8;
+      if (p.out_nt)
+        buf_st<2>(z, rout, off);
+      else
+        buf_st<0>(z, rout, off);
+    } else {
+      st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, z, p.out_nt);
+    }
+  };
 
   cx<T> v[E];
   if (active) {
@@ -225,8 +315,8 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const int k = t + m * TPT;
-        cx<T> s0 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)k * p.in_js, p.in_nt);
-        cx<T> s1 = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(L - k) * p.in_js, p.in_nt);
+        cx<T> s0 = load(k);
+        cx<T> s1 = load(L - k);
         cx<T> z;
         if (k == 0) {
           z = {s0.x + s1.x, s0.x - s1.x};
@@ -242,7 +332,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
     } else {
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
-        cx<T> z = ld_elem(in_re, in_im, p.in_il, ib + (idx_t)(t + m * TPT) * p.in_js, p.in_nt);
+        cx<T> z = load(t + m * TPT);
         v[m] = p.swap_in ? swapri(z) : z;
       });
     }
@@ -266,8 +356,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
         cx<T> x;
         if (k == 0) {
           x = {s0.x + s0.y, T(0)};
-          cx<T> xn = {s0.x - s0.y, T(0)};
-          st_elem(out_re, out_im, p.out_il, ob + (idx_t)L * p.out_ks, p.swap_out ? swapri(xn) : xn, p.out_nt);
+          store(L, cx<T>{s0.x - s0.y, T(0)});
         } else {
           cx<T> s1c = conjg(lat(L - k));
           cx<T> w = p.real_tw[k];
@@ -278,7 +367,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           cx<T> bb = {bw.x * h, bw.y * h};
           x = a + mul_neg_i(bb);
         }
-        st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, p.swap_out ? swapri(x) : x, p.out_nt);
+        store(k, x);
       });
     }
   } else {
@@ -301,29 +390,34 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       const int t2 = tid / CT;
       const int colb = col0 + c2;
       if (colb < p.n_col) {
-        const idx_t ob2 = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
+        // re-aim the output at column colb (the thread's column changes here)
+        if constexpr (FAST) {
+          out0 = (unsigned)colb * (unsigned)p.out_cs * ES;
+        } else {
+          ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
+          out_re = p.out_re;
+          out_im = p.out_im;
+        }
         static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
           const int k = t2 + m * TPT;
-          cx<T> z = lds[c2 * LL + lds_pad(k)];
-          st_elem(p.out_re, p.out_im, p.out_il, ob2 + (idx_t)k * p.out_ks, p.swap_out ? swapri(z) : z, p.out_nt);
+          store(k, lds[c2 * LL + lds_pad(k)]);
         });
       }
     } else if (active) {
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
-        cx<T> z = v[m];
-        st_elem(out_re, out_im, p.out_il, ob + (idx_t)(t + m * TPT) * p.out_ks, p.swap_out ? swapri(z) : z, p.out_nt);
+        store(t + m * TPT, v[m]);
       });
     }
   }
 }
 
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T>
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST>
 __global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
   const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
-  fft_tile<T, L, CT, COLMAP, EPI, STORE_T>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
+  fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
 }  // namespace afgpu

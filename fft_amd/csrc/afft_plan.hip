@@ -36,6 +36,7 @@
 #include "../../include/afft_gpu.h"
 #include "../../include/fft.h"
 #include "afft_aux.h"
+#include "afft_mfma.h"
 #include "afft_registry.h"
 
 using namespace afgpu;
@@ -96,18 +97,21 @@ struct PassRec {
   idx_t rr_h = 0, rr_m = 0, rr_rs = 0;
   int rr_inv = 0;
   int in_nt = 0, out_nt = 0, xcd = 0;
+  int mfma = 0;                  // run on the split-precision MFMA DFT-256 tile
+  const void* mfma_mats = nullptr;
 };
 
 struct Variant {
   int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
   int rowt_tile = 0;  // rows per group of transposed-store passes (0 = default)
-  int stream_io = 0;  // non-temporal loads of src / stores of dst
+  int stream_io = 1;  // non-temporal loads of src / stores of dst (keeps the MALL for scratch)
   int xcd_order = 0;  // XCD-contiguous tile order
   idx_t chunk = 0;    // transforms per pass group (0 = default)
 };
 
 struct Plan {
   Variant var;
+  int mode = 0;  // afft_gpu_mode_*
   int prec = 32;
   int kind = 0;
   int format = 1;
@@ -183,6 +187,9 @@ const void* get_table(int type, idx_t n, int bits) {
     size_t bytes = sizeof(cx<T>) * (size_t)(off > 0 ? off : 1);
     e = hipMalloc(&p, bytes);
     if (e == hipSuccess) e = gen_stage_tw<T>((cx<T>*)p, sl, nullptr);
+  } else if (type == 3) {  // A operands of the MFMA DFT-256 tile
+    e = hipMalloc(&p, mfma_mats_bytes());
+    if (e == hipSuccess) e = gen_mfma_mats(p, nullptr);
   } else if (type == 1) {  // four-step twiddles, hi then lo
     idx_t nhi = n >> bits, nlo = idx_t(1) << bits;
     e = hipMalloc(&p, sizeof(cx<T>) * (size_t)(nhi + nlo));
@@ -337,6 +344,7 @@ struct Builder {
     idx_t n2, n1;
     if (inner == 1) {
       n2 = n <= (idx_t)1024 * max_col_len<T>() ? 1024 : n / max_col_len<T>();
+      if (P.mode == afft_gpu_mode_mfma_f16 && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
       if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
         set_err("transform length %lld is too long", (long long)n);
         ok = false;
@@ -465,6 +473,13 @@ struct Builder {
       q.in_nt = P.var.stream_io && q.in.role == R_SRC;
       q.out_nt = P.var.stream_io && q.out.role == R_DST;
       q.xcd = P.var.xcd_order;
+      q.mfma = 0;
+      if (P.mode == afft_gpu_mode_mfma_f16 && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
+          (q.epi == EPI_PLAIN || q.epi == EPI_TW4) && q.in.il && q.out.il) {
+        q.mfma = 1;
+        q.mfma_mats = get_table<T>(3, 256, 0);
+        if (!q.mfma_mats) ok = false;
+      }
     }
     return r && ok;
   }
@@ -674,6 +689,42 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
   resolve(r.out, B, start, ore_c, oim_c);
   T* ore = const_cast<T*>(ore_c);
   T* oim = const_cast<T*>(oim_c);
+  if (r.type == PT_FFT && r.mfma) {
+    MfmaParams m{};
+    m.in = (const float*)ire;
+    m.out = (float*)ore;
+    m.in_js = r.in.s_j;
+    m.in_cs = r.in.s_c;
+    m.in_bs = r.in.s_b;
+    m.in_bs2 = r.in.s_b2;
+    m.out_ks = r.out.s_j;
+    m.out_cs = r.out.s_c;
+    m.out_bs = r.out.s_b;
+    m.out_bs2 = r.out.s_b2;
+    m.n_blo = r.n_blo;
+    idx_t ncol = r.fold_user ? r.n_col * cnt : r.n_col;
+    idx_t nb = r.fold_user ? 1 : r.b_per_user * cnt;
+    idx_t tiles = (ncol + 15) / 16;
+    if (ncol > 0x7fffffff || tiles * nb > 0x7fffffff) {
+      set_err("launch too large");
+      return false;
+    }
+    m.n_col = (int)ncol;
+    m.col_tiles = (int)tiles;
+    m.swap_in = r.swap_in;
+    m.swap_out = r.swap_out;
+    m.in_cfast = r.in.s_c == 1;
+    m.out_cfast = r.out.s_c == 1;
+    m.mats = (const h8*)r.mfma_mats;
+    if (r.epi == EPI_TW4) {
+      m.tw4_hi = (const cx<float>*)r.tw4_hi;
+      m.tw4_lo = (const cx<float>*)r.tw4_lo;
+      m.tw4_bits = r.tw4_bits;
+      m.tw4_shift = r.tw4_shift;
+    }
+    HIP_OK(launch_mfma_dft256(m, (int)(tiles * nb), st));
+    return true;
+  }
   if (r.type == PT_FFT) {
     const PassVariant<T>* v = (const PassVariant<T>*)r.variant;
     PassParams<T> p{};
@@ -713,7 +764,17 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     p.tw4_lo_bits = r.tw4_bits;
     p.tw4_col_shift = r.tw4_shift;
     p.real_tw = (const cx<T>*)r.real_tw;
-    HIP_OK(v->launch(p, st));
+    // byte extent touched inside one batch entry (both sides); the buffer
+    // path needs it below 4 GiB
+    const idx_t lin = (idx_t)r.L + (r.epi == EPI_C2R ? 1 : 0);
+    const idx_t lout = (idx_t)r.L + (r.epi == EPI_R2C ? 1 : 0);
+    const idx_t ext_in = ((ncol - 1) * r.in.s_c + (lin - 1) * r.in.s_j + 1) * (idx_t)sizeof(cx<T>);
+    const idx_t ext_out = ((ncol - 1) * r.out.s_c + (lout - 1) * r.out.s_j + 1) * (idx_t)sizeof(cx<T>);
+    const bool fast = r.in.il && r.out.il && ext_in > 0 && ext_out > 0 && ext_in < ((idx_t)1 << 32) &&
+                      ext_out < ((idx_t)1 << 32);
+    p.in_bytes = fast ? (unsigned)ext_in : 0u;
+    p.out_bytes = fast ? (unsigned)ext_out : 0u;
+    HIP_OK(v->launch(p, st, fast));
     return true;
   }
   if (r.type == PT_REALROWS) {
@@ -996,7 +1057,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   }
   for (int col : {0, 8})
     for (int rowt : {0, 8})
-      for (int sio : {0, 1})
+      for (int sio : {1, 0})
         for (int xcd : {0, 1})
           for (int cm : {1, 2, 4}) {
             Variant v;
@@ -1226,6 +1287,35 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   return 0;
 }
 
+int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode) {
+  g_err.clear();
+  Header h;
+  if (!read_header(plan, h)) return -1;
+  Plan& P = *h.plan;
+  if (mode != afft_gpu_mode_fma && mode != afft_gpu_mode_mfma_f16) {
+    set_err("unknown mode %d", mode);
+    return -1;
+  }
+  const int old = P.mode;
+  P.mode = mode;
+  if (!rebuild_any(P)) {
+    P.mode = old;
+    rebuild_any(P);
+    return -1;
+  }
+  if (mode == afft_gpu_mode_mfma_f16) {
+    int used = 0;
+    for (int i = 0; i < P.npass; i++) used += P.pass[i].mfma;
+    if (!used) {
+      set_err("no pass of this plan can run on the MFMA tile (needs FP32 c2c, interleaved, 256-point passes)");
+      P.mode = old;
+      rebuild_any(P);
+      return -1;
+    }
+  }
+  return 0;
+}
+
 int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   Header h;
   if (!read_header(plan, h) || !v) return -1;
@@ -1270,8 +1360,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   const PassRec& r = P.pass[i];
   static const char* ep[] = {"plain", "tw4", "r2c", "c2r"};
   if (r.type == PT_FFT)
-    snprintf(buf, len, "fft L=%d CT=%d %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.L, r.CT, r.colmap ? "col" : "row", ep[r.epi], r.store_t ? " store_t" : "", (long long)r.n_col,
+    snprintf(buf, len, "fft%s L=%d CT=%d %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
+             r.mfma ? "[mfma]" : "", r.L, r.CT, r.colmap ? "col" : "row", ep[r.epi], r.store_t ? " store_t" : "", (long long)r.n_col,
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);

@@ -9,7 +9,7 @@
 namespace afgpu {
 
 template <typename T>
-using PassLaunchFn = hipError_t (*)(const PassParams<T>&, hipStream_t);
+using PassLaunchFn = hipError_t (*)(const PassParams<T>&, hipStream_t, bool fast);
 
 template <typename T>
 struct PassVariant {

@@ -26,6 +26,13 @@ enum { afft_gpu = 16 };
 /* transform kinds */
 enum { afft_gpu_c = 0, afft_gpu_ci = 1, afft_gpu_r = 2, afft_gpu_ri = 3 };
 
+/* arithmetic modes of FP32 complex plans */
+enum
+{
+  afft_gpu_mode_fma = 0,      /* FP32 FMA Stockham passes (default) */
+  afft_gpu_mode_mfma_f16 = 1  /* 256-point passes as FP16 hi+lo MFMA products, FP32 accumulation */
+};
+
 typedef struct afft_gpu_plan_struct afft_gpu_plan;
 
 /* Bytes of host memory a plan header needs (any alignment). */
@@ -67,6 +74,17 @@ int afft_gpu_set_stream(void* state, void* stream);
 /* Number of transforms processed per pass group in large four-step plans
  * (0 = automatic).  Tuning knob; the result does not depend on it. */
 int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk);
+
+/* Selects the arithmetic mode.  afft_gpu_mode_mfma_f16 runs every 256-point
+ * pass of an FP32 interleaved c2c plan (1-D N = 256 or 65536 = 256 x 256, and
+ * 256-long axes of N-D shapes) on the matrix cores: each radix-16 stage is a
+ * product of FP16 hi+lo operands (per-butterfly power-of-two scaling) with
+ * FP32 accumulation.  Returns -1 if no pass of the plan qualifies. */
+int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode);
+
+/* Runs a device check of the MFMA operand layout the mode relies on; returns
+ * the number of mismatching entries (0 = correct) or -1. */
+int afft_gpu_mfma_selftest(void);
 
 /* Kernel-shape variant of a plan.  Every variant computes the same result;
  * they differ in speed.  0 means "default" for the integer fields. */
