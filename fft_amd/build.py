@@ -50,8 +50,17 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # the afft_run harness (host code; links the library, rpath to its folder)
+    harness = os.path.join(PKG, "afft_run")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-result", "-I", os.path.join(PKG, "..", "include"),
+           os.path.join(PKG, "harness", "afft_run.cpp"), "-L", PKG, "-lafft_mi355x", "-Wl,-rpath,$ORIGIN",
+           "-o", harness + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"afft_run build failed:\n{r.stderr[-4000:]}")
+    os.replace(harness + ".tmp", harness)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", harness)
     return LIB
 
 

@@ -177,57 +177,58 @@ struct Dft {
 // ---- per-length stage schedule ----------------------------------------------
 // L = R0 * R1 * ... ; every thread holds E = max radix elements; each stage of
 // radix R performs E/R butterflies per thread.  TPT = L / E threads per
-// transform.
-template <int L>
+// transform.  SV selects an alternative schedule for the same length (SV = 1:
+// fewer, larger radices with more elements per thread -- one LDS exchange less,
+// half the threads per transform).
+template <int L, int SV = 0>
 struct Sched;
-#define AFGPU_SCHED(LEN, EE, NSTAGE, ...)                          \
+#define AFGPU_SCHED(LEN, SVV, EE, NSTAGE, ...)                     \
   template <>                                                      \
-  struct Sched<LEN> {                                              \
+  struct Sched<LEN, SVV> {                                         \
     static constexpr int E = EE;                                   \
     static constexpr int NST = NSTAGE;                             \
     static constexpr int R[NSTAGE] = {__VA_ARGS__};                \
   };
-AFGPU_SCHED(1, 1, 1, 1)
-AFGPU_SCHED(2, 2, 1, 2)
-AFGPU_SCHED(4, 4, 1, 4)
-AFGPU_SCHED(8, 8, 1, 8)
-AFGPU_SCHED(16, 16, 1, 16)
-AFGPU_SCHED(32, 8, 2, 8, 4)
-AFGPU_SCHED(64, 8, 2, 8, 8)
-AFGPU_SCHED(128, 16, 2, 16, 8)
-AFGPU_SCHED(256, 16, 2, 16, 16)
-AFGPU_SCHED(512, 8, 3, 8, 8, 8)
-AFGPU_SCHED(1024, 16, 3, 16, 16, 4)
-AFGPU_SCHED(2048, 16, 3, 16, 16, 8)
-AFGPU_SCHED(4096, 16, 3, 16, 16, 16)
-AFGPU_SCHED(8192, 16, 4, 16, 16, 16, 2)
-AFGPU_SCHED(16384, 16, 4, 16, 16, 16, 4)
+AFGPU_SCHED(1, 0, 1, 1, 1)
+AFGPU_SCHED(2, 0, 2, 1, 2)
+AFGPU_SCHED(4, 0, 4, 1, 4)
+AFGPU_SCHED(8, 0, 8, 1, 8)
+AFGPU_SCHED(16, 0, 16, 1, 16)
+AFGPU_SCHED(32, 0, 8, 2, 8, 4)
+AFGPU_SCHED(64, 0, 8, 2, 8, 8)
+AFGPU_SCHED(128, 0, 16, 2, 16, 8)
+AFGPU_SCHED(256, 0, 16, 2, 16, 16)
+AFGPU_SCHED(512, 0, 8, 3, 8, 8, 8)
+AFGPU_SCHED(1024, 0, 16, 3, 16, 16, 4)
+AFGPU_SCHED(2048, 0, 16, 3, 16, 16, 8)
+AFGPU_SCHED(4096, 0, 16, 3, 16, 16, 16)
+AFGPU_SCHED(8192, 0, 16, 4, 16, 16, 16, 2)
+AFGPU_SCHED(16384, 0, 16, 4, 16, 16, 16, 4)
+AFGPU_SCHED(512, 1, 32, 2, 32, 16)
+AFGPU_SCHED(1024, 1, 32, 2, 32, 32)
+AFGPU_SCHED(2048, 1, 32, 3, 32, 32, 2)
 #undef AFGPU_SCHED
 
-template <int L, int S>
+template <int L, int SV, int S>
 struct StageNs {  // product of the radices before stage S
-  static constexpr int value = StageNs<L, S - 1>::value * Sched<L>::R[S - 1];
+  static constexpr int value = StageNs<L, SV, S - 1>::value * Sched<L, SV>::R[S - 1];
 };
-template <int L>
-struct StageNs<L, 0> {
+template <int L, int SV>
+struct StageNs<L, SV, 0> {
   static constexpr int value = 1;
 };
 
 // Offset (in complex entries) of stage S's twiddles in the per-length stage
 // table.  Stage S (S >= 1) has Ns * (R - 1) entries laid out [k][r - 1] with
 // value W_{Ns*R}^{r*k}.
-template <int L, int S>
+template <int L, int SV, int S>
 struct StageTwOff {
-  static constexpr int value =
-      StageTwOff<L, S - 1>::value + (S - 1 >= 1 ? StageNs<L, S - 1>::value * (Sched<L>::R[S - 1] - 1) : 0);
+  static constexpr int value = StageTwOff<L, SV, S - 1>::value +
+                               (S - 1 >= 1 ? StageNs<L, SV, S - 1>::value * (Sched<L, SV>::R[S - 1] - 1) : 0);
 };
-template <int L>
-struct StageTwOff<L, 0> {
+template <int L, int SV>
+struct StageTwOff<L, SV, 0> {
   static constexpr int value = 0;
-};
-template <int L>
-struct StageTwTotal {
-  static constexpr int value = StageTwOff<L, Sched<L>::NST>::value;
 };
 
 // LDS padding: one spare slot every 16 entries keeps the stride-R butterfly

@@ -127,15 +127,18 @@ __device__ __forceinline__ cx<T> tw4_lookup(const cx<T>* __restrict__ hi, const 
   return {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
 }
 
-template <typename T, int L, int S, typename LdsAt>
-__device__ __forceinline__ void run_stages(cx<T> (&v)[Sched<L>::E], int t, LdsAt& lat,
-                                           const cx<T>* __restrict__ tw) {
-  constexpr int E = Sched<L>::E;
+// The E registers of one thread.
+template <typename T, int L, int SV>
+using Regs = cx<T>[Sched<L, SV>::E];
+
+template <typename T, int L, int SV, int S, typename LdsAt>
+__device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat, const cx<T>* __restrict__ tw) {
+  constexpr int E = Sched<L, SV>::E;
   constexpr int TPT = L / E;
-  constexpr int R = Sched<L>::R[S];
-  constexpr int NS = StageNs<L, S>::value;
+  constexpr int R = Sched<L, SV>::R[S];
+  constexpr int NS = StageNs<L, SV, S>::value;
   constexpr int NB = E / R;
-  constexpr bool LAST = (S == Sched<L>::NST - 1);
+  constexpr bool LAST = (S == Sched<L, SV>::NST - 1);
   static_for<NB>([&](auto bc) AFGPU_INLINE {
     constexpr int b = decltype(bc)::value;
     cx<T> a[R];
@@ -143,7 +146,7 @@ __device__ __forceinline__ void run_stages(cx<T> (&v)[Sched<L>::E], int t, LdsAt
     if constexpr (NS > 1) {
       const int j = t + b * TPT;
       const int k = j & (NS - 1);
-      const cx<T>* tp = tw + StageTwOff<L, S>::value + k * (R - 1);
+      const cx<T>* tp = tw + StageTwOff<L, SV, S>::value + k * (R - 1);
       static_for<R - 1>([&](auto rc) AFGPU_INLINE {
         constexpr int r = decltype(rc)::value + 1;
         a[r] = cmul(a[r], tp[r - 1]);
@@ -168,13 +171,13 @@ __device__ __forceinline__ void run_stages(cx<T> (&v)[Sched<L>::E], int t, LdsAt
       v[m] = lat(t + m * TPT);
     });
     __syncthreads();
-    run_stages<T, L, S + 1>(v, t, lat, tw);
+    run_stages<T, L, SV, S + 1>(v, t, lat, tw);
   }
 }
 
-template <int L, int CT>
+template <int L, int CT, int SV = 0>
 struct PassShape {
-  static constexpr int E = Sched<L>::E;
+  static constexpr int E = Sched<L, SV>::E;
   static constexpr int TPT = L / E;
   static constexpr int NT = CT * TPT;
   static constexpr int LL = LdsLen<L>::value;
@@ -233,9 +236,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
 
 // One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
 // tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false>
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
-  using PS = PassShape<L, CT>;
+  using PS = PassShape<L, CT, SV>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
   constexpr int LL = PS::LL;
@@ -340,7 +343,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
     static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
   }
 
-  run_stages<T, L, 0>(v, t, lat, p.stage_tw);
+  run_stages<T, L, SV, 0>(v, t, lat, p.stage_tw);
 
   if constexpr (EPI == EPI_R2C) {
     static_for<E>([&](auto mc) AFGPU_INLINE {
@@ -371,6 +374,27 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       });
     }
   } else {
+    if constexpr (EPI == EPI_TW4 && !STORE_T) {
+      // twiddle and store four bins at a time: the scheduling barrier keeps
+      // the table loads of later groups from being hoisted, which would hold
+      // every twiddle in registers at once
+      const unsigned n2 = (unsigned)(col >> p.tw4_col_shift);
+      const unsigned mask = (1u << p.tw4_lo_bits) - 1u;
+      static_for<(E + 3) / 4>([&](auto gc) AFGPU_INLINE {
+        constexpr int g = decltype(gc)::value;
+        static_for<(E < 4 ? E : 4)>([&](auto ic) AFGPU_INLINE {
+          constexpr int m = 4 * g + decltype(ic)::value;
+          const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
+          const cx<T> h = p.tw4_hi[mi >> p.tw4_lo_bits];
+          const cx<T> l = p.tw4_lo[mi & mask];
+          const cx<T> w = {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
+          v[m] = cmul(v[m], w);
+          if (active) store(t + m * TPT, v[m]);
+        });
+        __builtin_amdgcn_sched_barrier(0);
+      });
+      return;
+    }
     if constexpr (EPI == EPI_TW4) {
       const idx_t n2 = (idx_t)(col >> p.tw4_col_shift);
       static_for<E>([&](auto mc) AFGPU_INLINE {
@@ -413,11 +437,11 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   }
 }
 
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST>
-__global__ void __launch_bounds__((PassShape<L, CT>::NT)) fft_pass_kernel(PassParams<T> p) {
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST, int SV>
+__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT)) fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
   const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
-  fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
+  fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST, SV>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
 }  // namespace afgpu

@@ -97,6 +97,7 @@ struct PassRec {
   idx_t rr_h = 0, rr_m = 0, rr_rs = 0;
   int rr_inv = 0;
   int in_nt = 0, out_nt = 0, xcd = 0;
+  int sv = 0;                    // stage schedule variant of the kernel
   int mfma = 0;                  // run on the split-precision MFMA DFT-256 tile
   const void* mfma_mats = nullptr;
 };
@@ -107,6 +108,7 @@ struct Variant {
   int stream_io = 1;  // non-temporal loads of src / stores of dst (keeps the MALL for scratch)
   int xcd_order = 0;  // XCD-contiguous tile order
   idx_t chunk = 0;    // transforms per pass group (0 = default)
+  int sched = 0;      // 1: radix-32 schedules for the long passes where compiled
 };
 
 struct Plan {
@@ -173,7 +175,7 @@ const void* get_table(int type, idx_t n, int bits) {
   hipError_t e = hipSuccess;
   if (type == 0) {  // stage twiddles of an n-point pass
     int radix[8], ns[8];
-    int nst = stage_list((int)n, radix, ns);
+    int nst = stage_list((int)n, bits, radix, ns);  // `bits` carries the schedule variant
     StageList sl{};
     sl.nstage = nst;
     int off = 0;
@@ -243,8 +245,11 @@ struct Builder {
   bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t) {
     int CT = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
     const PassVariant<T>* v = nullptr;
-    if (colmap && P.var.col_tile > 0) v = find_pass<T>(L, P.var.col_tile, colmap, epi, store_t);
-    if (store_t && P.var.rowt_tile > 0) v = find_pass<T>(L, P.var.rowt_tile, colmap, epi, store_t);
+    const int sv = P.var.sched;
+    if (sv && colmap) v = find_pass<T>(L, P.var.col_tile > 0 ? P.var.col_tile : 0, colmap, epi, store_t, sv);
+    if (sv && store_t && !v) v = find_pass<T>(L, P.var.rowt_tile > 0 ? P.var.rowt_tile : 0, colmap, epi, store_t, sv);
+    if (!v && colmap && P.var.col_tile > 0) v = find_pass<T>(L, P.var.col_tile, colmap, epi, store_t);
+    if (!v && store_t && P.var.rowt_tile > 0) v = find_pass<T>(L, P.var.rowt_tile, colmap, epi, store_t);
     if (v) CT = v->CT;
     if (!v) v = find_pass<T>(L, CT, colmap, epi, store_t);
     if (!v) {
@@ -259,7 +264,8 @@ struct Builder {
     r->epi = epi;
     r->store_t = store_t;
     r->variant = v;
-    r->stage_tw = get_table<T>(0, L, 0);
+    r->sv = v->sv;
+    r->stage_tw = get_table<T>(0, L, v->sv);
     if (!r->stage_tw) ok = false;
     if (epi == EPI_R2C || epi == EPI_C2R) {
       r->real_tw = L >= 2 ? get_table<T>(2, L, 0) : get_table<T>(2, 1, 0);
@@ -992,7 +998,7 @@ TuneCache& tcache() {
 }
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
-//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk>
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched>
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1003,7 +1009,8 @@ void tcache_load_locked(TuneCache& c) {
   char key[512];
   Variant v;
   long long ch = 0;
-  while (fscanf(f, "%511s %d %d %d %d %lld", key, &v.col_tile, &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch) == 6) {
+  while (fscanf(f, "%511s %d %d %d %d %lld %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch,
+                &v.sched) == 7) {
     v.chunk = ch;
     c.best[key] = v;
   }
@@ -1029,8 +1036,8 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
-          (long long)v.chunk);
+  fprintf(f, "%s %d %d %d %d %lld %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
+          (long long)v.chunk, v.sched);
   fclose(f);
 }
 
@@ -1059,8 +1066,10 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
     for (int rowt : {0, 8})
       for (int sio : {1, 0})
         for (int xcd : {0, 1})
+          for (int sch : {0, 1})
           for (int cm : {1, 2, 4}) {
             Variant v;
+            v.sched = sch;
             v.col_tile = col;
             v.rowt_tile = rowt;
             v.stream_io = sio;
@@ -1106,41 +1115,43 @@ namespace afgpu {
 const void* queue_table(int type, idx_t n, int bits) { return get_table<float>(type, n, bits); }
 void queue_set_err(const char* msg) { set_err("%s", msg); }
 
-template <int L>
+template <int L, int SV>
 static int fill_stages(int* radix, int* ns) {
   int acc = 1;
-  for (int s = 0; s < Sched<L>::NST; s++) {
-    radix[s] = Sched<L>::R[s];
+  for (int s = 0; s < Sched<L, SV>::NST; s++) {
+    radix[s] = Sched<L, SV>::R[s];
     ns[s] = acc;
-    acc *= Sched<L>::R[s];
+    acc *= Sched<L, SV>::R[s];
   }
-  return Sched<L>::NST;
+  return Sched<L, SV>::NST;
 }
-int stage_list(int L, int* radix, int* ns) {
+int stage_list(int L, int sv, int* radix, int* ns) {
+  if (sv == 1) {
+    switch (L) {
+      case 512: return fill_stages<512, 1>(radix, ns);
+      case 1024: return fill_stages<1024, 1>(radix, ns);
+      case 2048: return fill_stages<2048, 1>(radix, ns);
+      default: return 0;
+    }
+  }
   switch (L) {
-    case 1: return fill_stages<1>(radix, ns);
-    case 2: return fill_stages<2>(radix, ns);
-    case 4: return fill_stages<4>(radix, ns);
-    case 8: return fill_stages<8>(radix, ns);
-    case 16: return fill_stages<16>(radix, ns);
-    case 32: return fill_stages<32>(radix, ns);
-    case 64: return fill_stages<64>(radix, ns);
-    case 128: return fill_stages<128>(radix, ns);
-    case 256: return fill_stages<256>(radix, ns);
-    case 512: return fill_stages<512>(radix, ns);
-    case 1024: return fill_stages<1024>(radix, ns);
-    case 2048: return fill_stages<2048>(radix, ns);
-    case 4096: return fill_stages<4096>(radix, ns);
-    case 8192: return fill_stages<8192>(radix, ns);
-    case 16384: return fill_stages<16384>(radix, ns);
+    case 1: return fill_stages<1, 0>(radix, ns);
+    case 2: return fill_stages<2, 0>(radix, ns);
+    case 4: return fill_stages<4, 0>(radix, ns);
+    case 8: return fill_stages<8, 0>(radix, ns);
+    case 16: return fill_stages<16, 0>(radix, ns);
+    case 32: return fill_stages<32, 0>(radix, ns);
+    case 64: return fill_stages<64, 0>(radix, ns);
+    case 128: return fill_stages<128, 0>(radix, ns);
+    case 256: return fill_stages<256, 0>(radix, ns);
+    case 512: return fill_stages<512, 0>(radix, ns);
+    case 1024: return fill_stages<1024, 0>(radix, ns);
+    case 2048: return fill_stages<2048, 0>(radix, ns);
+    case 4096: return fill_stages<4096, 0>(radix, ns);
+    case 8192: return fill_stages<8192, 0>(radix, ns);
+    case 16384: return fill_stages<16384, 0>(radix, ns);
     default: return 0;
   }
-}
-int stage_tw_len(int L) {
-  int radix[8], ns[8];
-  int n = stage_list(L, radix, ns), tot = 0;
-  for (int s = 1; s < n; s++) tot += ns[s] * (radix[s] - 1);
-  return tot;
 }
 }  // namespace afgpu
 
@@ -1279,6 +1290,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.stream_io = v->stream_io;
   P.var.xcd_order = v->xcd_order;
   P.var.chunk = v->chunk;
+  P.var.sched = v->sched;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1325,6 +1337,7 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->stream_io = P.var.stream_io;
   v->xcd_order = P.var.xcd_order;
   v->chunk = (int64_t)P.chunk;
+  v->sched = P.var.sched;
   return 0;
 }
 

@@ -70,7 +70,7 @@ struct QArgs {
 
 template <int L>
 struct QGroupCT {
-  static constexpr int value = kQThreads / (L / Sched<L>::E);
+  static constexpr int value = kQThreads / (L / Sched<L, 0>::E);
 };
 
 // Each task body is its own function: inlined into one loop they blow the

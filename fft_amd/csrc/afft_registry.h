@@ -13,7 +13,7 @@ using PassLaunchFn = hipError_t (*)(const PassParams<T>&, hipStream_t, bool fast
 
 template <typename T>
 struct PassVariant {
-  int L, CT, colmap, epi, store_t;
+  int L, CT, colmap, epi, store_t, sv;
   int threads;
   size_t lds_bytes;
   PassLaunchFn<T> launch;
@@ -21,7 +21,7 @@ struct PassVariant {
 
 // Looks up the compiled variant; returns nullptr when none exists.
 template <typename T>
-const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t);
+const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t, int sv = 0);
 
 // Largest L with a compiled row / column kernel, and the column tile (CT) used
 // for a column pass of length L.
@@ -36,9 +36,8 @@ int row_tile(int L);
 template <typename T>
 int rowt_tile(int L);  // transposed-store row pass (four-step pass B)
 
-// Stage-twiddle table length for an L-point pass (complex entries).
-int stage_tw_len(int L);
-// Fills host-side descriptors of the stages of L: radix and Ns per stage.
-int stage_list(int L, int* radix, int* ns);
+// Fills host-side descriptors of the stages of schedule (L, sv): radix and Ns
+// per stage; returns the number of stages.
+int stage_list(int L, int sv, int* radix, int* ns);
 
 }  // namespace afgpu

@@ -95,6 +95,8 @@ typedef struct
   int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
   int64_t chunk;     /* transforms per pass group of four-step plans */
+  int32_t sched;     /* 1: radix-32 stage schedules for the long passes where compiled */
+  int32_t pad_;
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);

@@ -19,11 +19,13 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 def schedules():
     txt = open(os.path.join(ROOT, "fft_amd", "csrc", "afft_device.h")).read()
     out = {}
-    for m in re.finditer(r"AFGPU_SCHED\((\d+),\s*(\d+),\s*(\d+),\s*([\d,\s]+)\)", txt):
-        L, E, nst = int(m.group(1)), int(m.group(2)), int(m.group(3))
-        radix = [int(x) for x in m.group(4).split(",")]
+    for m in re.finditer(r"AFGPU_SCHED\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*([\d,\s]+)\)", txt):
+        L, sv, E, nst = (int(m.group(i)) for i in range(1, 5))
+        radix = [int(x) for x in m.group(5).split(",")]
         assert len(radix) == nst and int(np.prod(radix)) == L
-        out[L] = (E, radix)
+        assert all(E % r == 0 for r in radix)
+        out[(L, sv)] = (E, radix)
+    assert len(out) >= 15
     return out
 
 
@@ -82,9 +84,10 @@ def tile_model(x, E, radix):
     return out
 
 
-@pytest.mark.parametrize("L", sorted(schedules()))
-def test_tile_stages_and_table(L):
-    E, radix = schedules()[L]
+@pytest.mark.parametrize("key", sorted(schedules()), ids=lambda k: f"L{k[0]}_sv{k[1]}")
+def test_tile_stages_and_table(key):
+    L = key[0]
+    E, radix = schedules()[key]
     rng = np.random.default_rng(L)
     x = rng.standard_normal(L) + 1j * rng.standard_normal(L)
     y = tile_model(x, E, radix)
