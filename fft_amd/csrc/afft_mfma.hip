@@ -100,24 +100,28 @@ __device__ __forceinline__ void split8(const float (&x)[8], int sh, h8& hi, h8& 
   }
 }
 
+struct AOps {
+  h8 a0h, a0l, a1h, a1l;  // M-tile 0 (Re rows) and 1 (Im rows), hi and lo
+};
+
+__device__ __forceinline__ AOps load_aops(const h8* __restrict__ mats, int mat, int lane) {
+  return {mats[mat_index(mat, 0, 0, lane)], mats[mat_index(mat, 0, 1, lane)], mats[mat_index(mat, 1, 0, lane)],
+          mats[mat_index(mat, 1, 1, lane)]};
+}
+
 // One group: 16 butterflies (B columns), operands already in registers.
-__device__ __forceinline__ void mfma_group(const h8* __restrict__ mats, int mat, const float (&x)[8], f4& re,
-                                           f4& im, int lane) {
+__device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f4& re, f4& im) {
   const int sh = block_shift(x);
   h8 bh, bl;
   split8(x, sh, bh, bl);
-  const h8 a0h = mats[mat_index(mat, 0, 0, lane)];
-  const h8 a0l = mats[mat_index(mat, 0, 1, lane)];
-  const h8 a1h = mats[mat_index(mat, 1, 0, lane)];
-  const h8 a1l = mats[mat_index(mat, 1, 1, lane)];
   // small cross terms first, the large hi*hi product last
   f4 z = {0.f, 0.f, 0.f, 0.f};
-  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bl, z, 0, 0, 0);
-  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0l, bh, re, 0, 0, 0);
-  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0h, bh, re, 0, 0, 0);
-  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bl, z, 0, 0, 0);
-  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1l, bh, im, 0, 0, 0);
-  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1h, bh, im, 0, 0, 0);
+  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a0h, bl, z, 0, 0, 0);
+  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a0l, bh, re, 0, 0, 0);
+  re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a0h, bh, re, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1h, bl, z, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1l, bh, im, 0, 0, 0);
+  im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1h, bh, im, 0, 0, 0);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     re[j] = ldexpf(re[j], -sh);
@@ -125,9 +129,12 @@ __device__ __forceinline__ void mfma_group(const h8* __restrict__ mats, int mat,
   }
 }
 
-// 16 transforms of 256 points per work-group of 256 threads (4 waves).
+// 16 transforms of 256 points per work-group of 256 threads (4 waves).  The
+// tile lives in LDS as separate re / im planes so each lane reads exactly the
+// components its B-operand quarter needs.
 __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
-  __shared__ cx<float> lds[kMCT * kMLP];
+  __shared__ float lre[kMCT * kMLP];
+  __shared__ float lim[kMCT * kMLP];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -139,7 +146,15 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   const idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2;
   const idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2;
 
-  // load: 16 x 256 complex into lds[c][j]
+  // the A operands of both stages, issued before the tile load so their
+  // latency overlaps it
+  const AOps A1 = load_aops(p.mats, 0, lane);
+  AOps A2[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
+
+  // load: 16 x 256 complex into the planes
+#pragma unroll 4
   for (int it = 0; it < kMCT; it++) {
     int c, j;
     if (p.in_cfast) {
@@ -155,64 +170,58 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
       v = reinterpret_cast<const cx<float>*>(p.in)[e];
       if (p.swap_in) v = swapri(v);
     }
-    lds[c * kMLP + j] = v;
+    lre[c * kMLP + j] = v.x;
+    lim[c * kMLP + j] = v.y;
   }
   __syncthreads();
 
   const int n = lane & 15;   // B column within the group
   const int q = lane >> 4;   // quarter of K
   const int rb = 8 * (q & 1);
-  const bool use_im = q >= 2;
+  const float* plane = q >= 2 ? lim : lre;
 
   // stage 1: wave w handles columns 4w..4w+3; group = column, B column = butterfly j
-#pragma unroll 1
+#pragma unroll
   for (int g = 0; g < 4; g++) {
     const int c = wave * 4 + g;
     const int j = n;
     float x[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      cx<float> v = lds[c * kMLP + j + 16 * (rb + i)];
-      x[i] = use_im ? v.y : v.x;
-    }
+    for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)];
     f4 re, im;
-    mfma_group(p.mats, 0, x, re, im, lane);
-    // every lane of the wave has read its inputs before the writes below
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_s_waitcnt(0);
+    mfma_group(A1, x, re, im);
+    // the MFMA consumed every lane's loads, so the column can be overwritten
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
       const int m = 4 * q + jj;
-      lds[c * kMLP + 16 * j + m] = cx<float>{re[jj], im[jj]};
+      lre[c * kMLP + 16 * j + m] = re[jj];
+      lim[c * kMLP + 16 * j + m] = im[jj];
     }
   }
   __syncthreads();
 
   // stage 2: wave w handles butterflies j = 4w..4w+3 of all 16 columns;
   // group = j (one twiddled matrix), B column = transform c
-#pragma unroll 1
+#pragma unroll
   for (int g = 0; g < 4; g++) {
     const int j = wave * 4 + g;
     const int c = n;
     float x[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      cx<float> v = lds[c * kMLP + j + 16 * (rb + i)];
-      x[i] = use_im ? v.y : v.x;
-    }
+    for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)];
     f4 re, im;
-    mfma_group(p.mats, 1 + j, x, re, im, lane);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_s_waitcnt(0);
+    mfma_group(A2[g], x, re, im);
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
       const int m = 4 * q + jj;
-      lds[c * kMLP + j + 16 * m] = cx<float>{re[jj], im[jj]};
+      lre[c * kMLP + j + 16 * m] = re[jj];
+      lim[c * kMLP + j + 16 * m] = im[jj];
     }
   }
   __syncthreads();
 
   // store (optional four-step twiddle W_N^(n2 * k1), n2 = column, k1 = bin)
+#pragma unroll 4
   for (int it = 0; it < kMCT; it++) {
     int c, k;
     if (p.out_cfast) {
@@ -223,7 +232,7 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
       k = tid;
     }
     if (col0 + c >= p.n_col) continue;
-    cx<float> v = lds[c * kMLP + k];
+    cx<float> v = {lre[c * kMLP + k], lim[c * kMLP + k]};
     if (p.tw4_hi) v = cmul(v, tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_bits, (idx_t)((col0 + c) >> p.tw4_shift) * k));
     if (p.swap_out) v = swapri(v);
     idx_t e = ob + (idx_t)(col0 + c) * p.out_cs + (idx_t)k * p.out_ks;
