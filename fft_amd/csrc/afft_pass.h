@@ -127,6 +127,15 @@ __device__ __forceinline__ cx<T> tw4_lookup(const cx<T>* __restrict__ hi, const 
   return {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
 }
 
+// Same with a 32-bit index (m < 2^32) and a precomputed mask.
+template <typename T>
+__device__ __forceinline__ cx<T> tw4_lookup32(const cx<T>* __restrict__ hi, const cx<T>* __restrict__ lo, int bits,
+                                              unsigned mask, unsigned m) {
+  cx<T> h = hi[m >> bits];
+  cx<T> l = lo[m & mask];
+  return {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
+}
+
 // The E registers of one thread.
 template <typename T, int L, int SV>
 using Regs = cx<T>[Sched<L, SV>::E];
@@ -385,9 +394,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
         static_for<(E < 4 ? E : 4)>([&](auto ic) AFGPU_INLINE {
           constexpr int m = 4 * g + decltype(ic)::value;
           const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
-          const cx<T> h = p.tw4_hi[mi >> p.tw4_lo_bits];
-          const cx<T> l = p.tw4_lo[mi & mask];
-          const cx<T> w = {h.x + fma(h.x, l.x, -(h.y * l.y)), h.y + fma(h.x, l.y, h.y * l.x)};
+          const cx<T> w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
           v[m] = cmul(v[m], w);
           if (active) store(t + m * TPT, v[m]);
         });
