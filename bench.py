@@ -13,7 +13,11 @@ The JSON line also carries
   cpu_baseline  afft itself (oracle/_ref, built from the reference sources) on
                 this host's cores over a bounded sample of the same workload;
   extra         the rest of the afft configs measured the same way (r2c
-                16 x 2^22, c2c 256^3, r2c 8192^2, the device-copy ceiling).
+                16 x 2^22, c2c 256^3, r2c 8192^2, the device-copy ceiling),
+                the MFMA split-precision mode against the FP32-FMA plan on
+                4096 x 2^16 with 10^u amplitudes (time and error vs FP64),
+                and the 10,000-job uneven batch through the persistent work
+                queue (eager and graph-replay p50/p99).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
@@ -105,6 +109,104 @@ def cpu_baseline(seconds: float):
         res[nthr] = (flops_c2c(n, total) / el / 1e9, total, el)
     isa = "avx512f" if R.supported(O.AFFT_AVX512F) else "avx2" if R.supported(O.AFFT_AVX2) else "scalar"
     return res, threads, isa
+
+
+def bench_mfma(torch, dev, stream, g, iters=5):
+    """4096 c2c transforms of N=2^16, each scaled by 10^u, u ~ U[-3, 3):
+    plain FP32-FMA plan against the split-precision MFMA mode, both timed on
+    the same buffers, both checked per transform against the device FP64
+    plan (max rel-L2 over the 4096 transforms, bound 2*16*2^-24)."""
+    from fft_amd.afft import Plan
+    n, batch = 1 << 16, 4096
+    u = torch.rand(batch, 1, device=dev, generator=g, dtype=torch.float64) * 6 - 3
+    x = (torch.rand(batch, 2 * n, device=dev, generator=g, dtype=torch.float64) * 2 - 1) * torch.pow(10.0, u)
+    x32 = x.float().contiguous()
+    x64 = x32.double().contiguous()
+    del x, u
+    y = torch.empty_like(x32)
+    ref = torch.empty_like(x64)
+    p64 = Plan("c", (n,), batch, dtype=np.float64)
+    p64(x64, None, ref, None, stream)
+    del x64
+    pf = Plan("c", (n,), batch)
+    pm = Plan("c", (n,), batch)
+    pm.set_mode(1)
+    out = {}
+    bound = 2.0 * 16 * 2.0 ** -24
+    for tag, p in (("fp32", pf), ("mfma", pm)):
+        p(x32, None, y, None, stream)
+        ms = time_events(torch, stream, lambda: p(x32, None, y, None, stream), iters)
+        d = (y.double() - ref).pow_(2).sum(1)
+        r = ref.pow(2).sum(1)
+        err = torch.sqrt(d / r)
+        out[tag] = (ms, float(err.max()), float(err.mean()))
+    del y, ref, x32, pf, pm, p64
+    torch.cuda.empty_cache()
+    (mf, ef, af), (mm, em, am) = out["fp32"], out["mfma"]
+    return {"workload": "4096 x c2c N=2^16 FP32 interleaved, amplitudes 10^u, u~U[-3,3) per transform",
+            "fp32_fma_ms": round(mf, 4), "mfma_ms": round(mm, 4), "mfma_speedup": round(mf / mm, 3),
+            "fp32_err_max": ef, "fp32_err_mean": af, "mfma_err_max": em, "mfma_err_mean": am,
+            "err_bound": bound, "mfma_GFLOPs": round(flops_c2c(n, batch) / (mm * 1e-3) / 1e9, 1),
+            "fp32_GFLOPs": round(flops_c2c(n, batch) / (mf * 1e-3) / 1e9, 1)}
+
+
+def bench_queue(torch, dev, stream, reps=40):
+    """10,000 1-D FP32 jobs, N = 2^k with k ~ U{6..16}, 70% complex / 30%
+    real, direction at random, in one persistent launch per batch; timed
+    eagerly and as a replayed HIP graph (per-batch latency p50/p99)."""
+    from fft_amd.afft import Queue
+    rng = np.random.default_rng(5)
+    lay, off = [], 0
+    for _ in range(10000):
+        lg = int(rng.integers(6, 17))
+        real = rng.random() < 0.3
+        inv = rng.random() < 0.5
+        n = 1 << lg
+        kind = (3 if inv else 2) if real else (1 if inv else 0)
+        nin = {0: 2 * n, 1: 2 * n, 2: n, 3: 2 * (n // 2 + 1)}[kind]
+        nout = {0: 2 * n, 1: 2 * n, 2: 2 * (n // 2 + 1), 3: n}[kind]
+        so = off
+        off += (nin + 63) // 64 * 64
+        do = off
+        off += (nout + 63) // 64 * 64
+        lay.append((kind, lg, so, do, nin, nout))
+    buf = torch.rand(off, device=dev, dtype=torch.float32)
+    base = int(buf.data_ptr())
+    q = Queue(10000, 16)
+    jobs = Queue.jobs([(k, lg, base + 4 * so, base + 4 * do) for (k, lg, so, do, _, _) in lay])
+    t0 = time.perf_counter()
+    ntask = q.prepare(jobs)
+    prep_ms = (time.perf_counter() - t0) * 1e3
+    q.launch(stream)
+    ms = time_events(torch, stream, lambda: q.launch(stream), 10)
+    algb = sum(4 * (nin + nout) for (_, _, _, _, nin, nout) in lay)
+    fl = sum((5.0 if k < 2 else 2.5) * (1 << lg) * lg for (k, lg, _, _, _, _) in lay)
+    s2 = torch.cuda.Stream()
+    gr = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s2):
+        with torch.cuda.graph(gr, stream=s2):
+            q.launch(s2)
+    torch.cuda.synchronize()
+    lat = []
+    for i in range(reps + 5):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        gr.replay()
+        b.record(stream)
+        torch.cuda.synchronize()
+        if i >= 5:
+            lat.append(a.elapsed_time(b))
+    lat = np.array(lat)
+    del gr
+    q.close()
+    del buf
+    torch.cuda.empty_cache()
+    return {"workload": "10000 x 1-D FP32 jobs, N=2^k k~U{6..16}, 70% c2c / 30% real, fwd/inv at random",
+            "tasks": int(ntask), "prepare_host_ms": round(prep_ms, 3), "launch_ms": round(ms, 4),
+            "GFLOPs": round(fl / (ms * 1e-3) / 1e9, 1), "algorithmic_GBps": round(algb / (ms * 1e-3) / 1e9, 1),
+            "graph_replay_p50_ms": round(float(np.percentile(lat, 50)), 4),
+            "graph_replay_p99_ms": round(float(np.percentile(lat, 99)), 4), "graph_replays": reps}
 
 
 def main():
@@ -203,6 +305,8 @@ def main():
         extra["r2c_8192^2_hbm_frac"] = round((4 * n2 + 8 * 4097 * 8192) / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
         del x2, y2, p2
         torch.cuda.empty_cache()
+        extra["mfma_vs_fp32"] = bench_mfma(torch, dev, stream, g)
+        extra["uneven_queue"] = bench_queue(torch, dev, stream)
 
     cpu = None
     if not args.no_cpu:

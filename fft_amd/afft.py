@@ -120,6 +120,17 @@ def align_size(n: int, dtype=np.float32) -> int:
     return int(getattr(_lib.lib(), f"afft_{p}_align_size")(int(n)))
 
 
+class Variant(ctypes.Structure):
+    """afft_gpu_variant (include/afft_gpu.h): kernel-shape choices of a plan;
+    0 = default for every field."""
+    _fields_ = [("col_tile", ctypes.c_int32), ("rowt_tile", ctypes.c_int32), ("stream_io", ctypes.c_int32),
+                ("xcd_order", ctypes.c_int32), ("chunk", ctypes.c_int64), ("sched", ctypes.c_int32),
+                ("half_lds", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Plan:
     """Batched asynchronous plan (afft_gpu_plan_create / afft_gpu_execute)."""
 
@@ -158,6 +169,31 @@ class Plan:
     def set_chunk(self, chunk: int):
         if _lib.lib().afft_gpu_plan_set_chunk(self.state, int(chunk)) != 0:
             raise RuntimeError(_lib.last_error())
+
+    def get_variant(self) -> Variant:
+        v = Variant()
+        if _lib.lib().afft_gpu_plan_get_variant(self.state, ctypes.byref(v)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return v
+
+    def set_variant(self, **fields):
+        """Rebuilds the plan with the given variant fields (others: default;
+        stream_io defaults to 1)."""
+        v = Variant(stream_io=1)
+        for k, val in fields.items():
+            setattr(v, k, int(val))
+        if _lib.lib().afft_gpu_plan_set_variant(self.state, ctypes.byref(v)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    def tune(self, src_re, src_im, dst_re, dst_im, stream=None, iters: int = 3) -> float:
+        """Runs the autotuner on these buffers; returns the best ms."""
+        h = getattr(stream, "cuda_stream", stream)
+        best = ctypes.c_float(0)
+        r = _lib.lib().afft_gpu_plan_tune(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re),
+                                          _lib.ptr(dst_im), h, int(iters), ctypes.byref(best))
+        if r != 0:
+            raise RuntimeError(_lib.last_error())
+        return best.value
 
     def set_mode(self, mode: int):
         if _lib.lib().afft_gpu_plan_set_mode(self.state, int(mode)) != 0:

@@ -140,7 +140,11 @@ __device__ __forceinline__ cx<T> tw4_lookup32(const cx<T>* __restrict__ hi, cons
 template <typename T, int L, int SV>
 using Regs = cx<T>[Sched<L, SV>::E];
 
-template <typename T, int L, int SV, int S, typename LdsAt>
+// Stage S and the ones after it.  HX (half exchange): `lat` addresses one real
+// per entry and each exchange between stages moves the real parts, then the
+// imaginary parts, through the same LDS words -- half the LDS of a tile for
+// two more barriers per exchange.
+template <typename T, int L, int SV, int S, bool HX, typename LdsAt>
 __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat, const cx<T>* __restrict__ tw) {
   constexpr int E = Sched<L, SV>::E;
   constexpr int TPT = L / E;
@@ -165,22 +169,46 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
     static_for<R>([&](auto rc) AFGPU_INLINE { v[b + decltype(rc)::value * NB] = a[decltype(rc)::value]; });
   });
   if constexpr (!LAST) {
-    static_for<NB>([&](auto bc) AFGPU_INLINE {
-      constexpr int b = decltype(bc)::value;
-      const int j = t + b * TPT;
-      const int base = (j / NS) * (NS * R) + (j & (NS - 1));
-      static_for<R>([&](auto rc) AFGPU_INLINE {
-        constexpr int r = decltype(rc)::value;
-        lat(base + r * NS) = v[b + r * NB];
+    if constexpr (!HX) {
+      static_for<NB>([&](auto bc) AFGPU_INLINE {
+        constexpr int b = decltype(bc)::value;
+        const int j = t + b * TPT;
+        const int base = (j / NS) * (NS * R) + (j & (NS - 1));
+        static_for<R>([&](auto rc) AFGPU_INLINE {
+          constexpr int r = decltype(rc)::value;
+          lat(base + r * NS) = v[b + r * NB];
+        });
       });
-    });
-    __syncthreads();
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      v[m] = lat(t + m * TPT);
-    });
-    __syncthreads();
-    run_stages<T, L, SV, S + 1>(v, t, lat, tw);
+      __syncthreads();
+      static_for<E>([&](auto mc) AFGPU_INLINE {
+        constexpr int m = decltype(mc)::value;
+        v[m] = lat(t + m * TPT);
+      });
+      __syncthreads();
+    } else {
+      static_for<2>([&](auto hc) AFGPU_INLINE {
+        constexpr int h = decltype(hc)::value;
+        static_for<NB>([&](auto bc) AFGPU_INLINE {
+          constexpr int b = decltype(bc)::value;
+          const int j = t + b * TPT;
+          const int base = (j / NS) * (NS * R) + (j & (NS - 1));
+          static_for<R>([&](auto rc) AFGPU_INLINE {
+            constexpr int r = decltype(rc)::value;
+            lat(base + r * NS) = h ? v[b + r * NB].y : v[b + r * NB].x;
+          });
+        });
+        __syncthreads();
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          if constexpr (h)
+            v[m].y = lat(t + m * TPT);
+          else
+            v[m].x = lat(t + m * TPT);
+        });
+        __syncthreads();
+      });
+    }
+    run_stages<T, L, SV, S + 1, HX>(v, t, lat, tw);
   }
 }
 
@@ -191,6 +219,11 @@ struct PassShape {
   static constexpr int NT = CT * TPT;
   static constexpr int LL = LdsLen<L>::value;
   static constexpr size_t lds_elems = size_t(CT) * LL;
+  // LDS bytes of a tile of element type T (HX: one real per entry)
+  template <typename T, bool HX>
+  static constexpr size_t lds_bytes() {
+    return lds_elems * (HX ? sizeof(T) : 2 * sizeof(T));
+  }
 };
 
 // One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
@@ -245,8 +278,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
 
 // One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
 // tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0>
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
+  static_assert(!(HX && EPI == EPI_R2C), "the real-output epilogue exchanges whole complex entries");
   using PS = PassShape<L, CT, SV>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
@@ -267,6 +301,13 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       return lds[lds_pad(i) * CT + c];
     else
       return lds[c * LL + lds_pad(i)];
+  };
+  T* __restrict__ ldsf = reinterpret_cast<T*>(lds);
+  auto latf = [&](int i) -> T& {
+    if constexpr (COLMAP)
+      return ldsf[lds_pad(i) * CT + c];
+    else
+      return ldsf[c * LL + lds_pad(i)];
   };
 
   // generic addressing
@@ -352,7 +393,10 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
     static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
   }
 
-  run_stages<T, L, SV, 0>(v, t, lat, p.stage_tw);
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true>(v, t, latf, p.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false>(v, t, lat, p.stage_tw);
 
   if constexpr (EPI == EPI_R2C) {
     static_for<E>([&](auto mc) AFGPU_INLINE {
@@ -412,28 +456,48 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
     }
     if constexpr (STORE_T) {
       static_assert(!COLMAP, "transposed store is for row-mapped passes");
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        lat(t + m * TPT) = v[m];
-      });
-      __syncthreads();
       const int c2 = tid % CT;
       const int t2 = tid / CT;
       const int colb = col0 + c2;
-      if (colb < p.n_col) {
-        // re-aim the output at column colb (the thread's column changes here)
-        if constexpr (FAST) {
-          out0 = (unsigned)colb * (unsigned)p.out_cs * ES;
-        } else {
-          ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
-          out_re = p.out_re;
-          out_im = p.out_im;
-        }
+      // re-aim the output at column colb (the thread's column changes here)
+      if constexpr (FAST) {
+        out0 = (unsigned)colb * (unsigned)p.out_cs * ES;
+      } else {
+        ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
+        out_re = p.out_re;
+        out_im = p.out_im;
+      }
+      if constexpr (HX) {
+        T xs[E];
+        static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
+        __syncthreads();
         static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
-          const int k = t2 + m * TPT;
-          store(k, lds[c2 * LL + lds_pad(k)]);
+          xs[m] = ldsf[c2 * LL + lds_pad(t2 + m * TPT)];
         });
+        __syncthreads();
+        static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
+        __syncthreads();
+        if (colb < p.n_col) {
+          static_for<E>([&](auto mc) AFGPU_INLINE {
+            constexpr int m = decltype(mc)::value;
+            const int k = t2 + m * TPT;
+            store(k, cx<T>{xs[m], ldsf[c2 * LL + lds_pad(k)]});
+          });
+        }
+      } else {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          lat(t + m * TPT) = v[m];
+        });
+        __syncthreads();
+        if (colb < p.n_col) {
+          static_for<E>([&](auto mc) AFGPU_INLINE {
+            constexpr int m = decltype(mc)::value;
+            const int k = t2 + m * TPT;
+            store(k, lds[c2 * LL + lds_pad(k)]);
+          });
+        }
       }
     } else if (active) {
       static_for<E>([&](auto mc) AFGPU_INLINE {
@@ -444,11 +508,12 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   }
 }
 
-template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST, int SV>
-__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT)) fft_pass_kernel(PassParams<T> p) {
+template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST, int SV, bool HX>
+__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (HX ? PassShape<L, CT, SV>::NT / 128 : 1)) fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
   const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
-  fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST, SV>(p, tile, threadIdx.x, reinterpret_cast<cx<T>*>(afgpu_smem));
+  fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST, SV, HX>(p, tile, threadIdx.x,
+                                                         reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
 }  // namespace afgpu

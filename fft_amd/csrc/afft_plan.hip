@@ -98,6 +98,7 @@ struct PassRec {
   int rr_inv = 0;
   int in_nt = 0, out_nt = 0, xcd = 0;
   int sv = 0;                    // stage schedule variant of the kernel
+  int hx = 0;                    // half-LDS kernel (real / imaginary parts exchanged in turn)
   int mfma = 0;                  // run on the split-precision MFMA DFT-256 tile
   const void* mfma_mats = nullptr;
 };
@@ -109,6 +110,7 @@ struct Variant {
   int xcd_order = 0;  // XCD-contiguous tile order
   idx_t chunk = 0;    // transforms per pass group (0 = default)
   int sched = 0;      // 1: radix-32 schedules for the long passes where compiled
+  int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
 };
 
 struct Plan {
@@ -257,6 +259,10 @@ struct Builder {
       ok = false;
       return false;
     }
+    if (P.var.half_lds >= 0) {
+      const PassVariant<T>* h = find_pass<T>(L, CT, colmap, epi, store_t, v->sv, 1);
+      if (h) v = h;
+    }
     r->type = PT_FFT;
     r->L = L;
     r->CT = CT;
@@ -265,6 +271,7 @@ struct Builder {
     r->store_t = store_t;
     r->variant = v;
     r->sv = v->sv;
+    r->hx = v->hx;
     r->stage_tw = get_table<T>(0, L, v->sv);
     if (!r->stage_tw) ok = false;
     if (epi == EPI_R2C || epi == EPI_C2R) {
@@ -998,7 +1005,7 @@ TuneCache& tcache() {
 }
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
-//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched>
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds>]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1006,11 +1013,13 @@ void tcache_load_locked(TuneCache& c) {
   if (!path) return;
   FILE* f = fopen(path, "r");
   if (!f) return;
-  char key[512];
-  Variant v;
-  long long ch = 0;
-  while (fscanf(f, "%511s %d %d %d %d %lld %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch,
-                &v.sched) == 7) {
+  char line[1024], key[512];
+  while (fgets(line, sizeof(line), f)) {
+    Variant v;
+    long long ch = 0;
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
+                           &v.xcd_order, &ch, &v.sched, &v.half_lds);
+    if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
   }
@@ -1036,8 +1045,8 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
-          (long long)v.chunk, v.sched);
+  fprintf(f, "%s %d %d %d %d %lld %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
+          (long long)v.chunk, v.sched, v.half_lds);
   fclose(f);
 }
 
@@ -1062,22 +1071,23 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
     if (!rebuild<T>(P)) return false;
     base_chunk = P.chunk;
   }
+  // the XCD order stays a manual knob: it has not won on any measured shape
   for (int col : {0, 8})
     for (int rowt : {0, 8})
       for (int sio : {1, 0})
-        for (int xcd : {0, 1})
+        for (int hl : {0, -1})
           for (int sch : {0, 1})
-          for (int cm : {1, 2, 4}) {
-            Variant v;
-            v.sched = sch;
-            v.col_tile = col;
-            v.rowt_tile = rowt;
-            v.stream_io = sio;
-            v.xcd_order = xcd;
-            v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
-            if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
-            cands.push_back(v);
-          }
+            for (int cm : {1, 2, 4}) {
+              Variant v;
+              v.sched = sch;
+              v.col_tile = col;
+              v.rowt_tile = rowt;
+              v.stream_io = sio;
+              v.half_lds = hl;
+              v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
+              if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
+              cands.push_back(v);
+            }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
   float best = 1e30f;
@@ -1291,6 +1301,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.xcd_order = v->xcd_order;
   P.var.chunk = v->chunk;
   P.var.sched = v->sched;
+  P.var.half_lds = v->half_lds;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1338,6 +1349,7 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->xcd_order = P.var.xcd_order;
   v->chunk = (int64_t)P.chunk;
   v->sched = P.var.sched;
+  v->half_lds = P.var.half_lds;
   return 0;
 }
 
@@ -1373,8 +1385,11 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   const PassRec& r = P.pass[i];
   static const char* ep[] = {"plain", "tw4", "r2c", "c2r"};
   if (r.type == PT_FFT)
-    snprintf(buf, len, "fft%s L=%d CT=%d %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.mfma ? "[mfma]" : "", r.L, r.CT, r.colmap ? "col" : "row", ep[r.epi], r.store_t ? " store_t" : "", (long long)r.n_col,
+    snprintf(buf, len,
+             "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
+             r.mfma ? "[mfma]" : "", r.L, r.CT, r.sv ? " sv1" : "",
+             r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
+             r.store_t ? " store_t" : "", (long long)r.n_col,
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);

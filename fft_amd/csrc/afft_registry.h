@@ -14,6 +14,7 @@ using PassLaunchFn = hipError_t (*)(const PassParams<T>&, hipStream_t, bool fast
 template <typename T>
 struct PassVariant {
   int L, CT, colmap, epi, store_t, sv;
+  int hx;  // half exchange: real and imaginary parts cross the LDS in turn
   int threads;
   size_t lds_bytes;
   PassLaunchFn<T> launch;
@@ -21,7 +22,7 @@ struct PassVariant {
 
 // Looks up the compiled variant; returns nullptr when none exists.
 template <typename T>
-const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t, int sv = 0);
+const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t, int sv = 0, int hx = 0);
 
 // Largest L with a compiled row / column kernel, and the column tile (CT) used
 // for a column pass of length L.

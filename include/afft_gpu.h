@@ -96,7 +96,8 @@ typedef struct
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
   int64_t chunk;     /* transforms per pass group of four-step plans */
   int32_t sched;     /* 1: radix-32 stage schedules for the long passes where compiled */
-  int32_t pad_;
+  int32_t half_lds;  /* 1: half-LDS kernels (real and imaginary parts exchanged in turn)
+                        where compiled, -1: never, 0: default (on) */
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);

@@ -224,3 +224,58 @@ def test_errors_leave_state_clean(G):
     mem = ctypes.create_string_buffer(64)
     assert not L.afft_32_c_create(nd, dims, 1, mem, 0)
     assert "power of two" in _lib.last_error()
+
+
+VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c)
+            for s in (0, 1) for h in (0, -1) for r in (0, 8) for c in (0, 8)]
+
+
+@pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
+                                          ("c", 1 << 17, 2), ("c", 1 << 14, 3), ("ci", 1 << 13, 3),
+                                          ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 1)])
+def test_kernel_variants_against_oracle(G, kind, n, batch):
+    """Every kernel-shape variant (radix-32 schedules, half-LDS exchange,
+    narrow tiles) against the FP64 oracle; the variants only change speed."""
+    import torch
+    from fft_amd.afft import Plan
+    say("variants", kind, n, batch)
+    real = kind in ("r", "ri")
+    if kind == "r":
+        x = O.normal(31, batch * n).reshape(batch, n).astype(np.float32)
+        ref = [O.rfft(x[b].astype(np.float64)) for b in range(batch)]
+        src = G.to_dev(x.ravel())
+        nout = 2 * (n // 2 + 1) * batch
+    elif kind == "ri":
+        z = np.stack([O.hermitian_input(32 + b, (n,)) for b in range(batch)])
+        ref = [O.irfft(z[b].astype(np.complex128), (n,)) for b in range(batch)]
+        src = G.to_dev(np.stack([z.real, z.imag], -1).astype(np.float32).ravel())
+        nout = n * batch
+    else:
+        x = O.complex_input(33, (batch, n))
+        ref = [O.fft(x[b].astype(np.complex128), inverse=(kind == "ci")) for b in range(batch)]
+        src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+        nout = 2 * n * batch
+    p = Plan(kind, (n,), batch)
+    seen = set()
+    for v in VARIANTS:
+        p.set_variant(**v)
+        desc = tuple(p.describe())
+        if desc in seen:
+            continue
+        seen.add(desc)
+        say("  variant", v, desc)
+        dst = G.empty_dev(nout)
+        p(src, None, dst, None)
+        torch.cuda.synchronize()
+        y = G.to_host(dst)
+        if kind == "ri":
+            y = y.reshape(batch, n)
+        elif kind == "r":
+            y = (y[0::2] + 1j * y[1::2]).reshape(batch, n // 2 + 1)
+        else:
+            y = (y[0::2] + 1j * y[1::2]).reshape(batch, n)
+        for b in range(batch):
+            yb = y[b].astype(np.float32 if kind == "ri" else np.complex64)
+            assert O.rel_l2(yb, ref[b]) <= O.bound(n), (kind, n, v, b)
+    assert len(seen) >= 2 or n < (1 << 13), seen
+    p.close()
