@@ -111,6 +111,22 @@ def cpu_baseline(seconds: float):
     return res, threads, isa
 
 
+def measured_traffic(describe: str, launch_transforms: int):
+    """HBM bytes per launch of the dominant pass from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/*_pmc.json, written by
+    fft_amd.profile_summary), if one matches this pass and launch size."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_pmc.json"))):
+        try:
+            with open(f) as fh:
+                js = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        if js.get("describe") == describe and int(js.get("launch_transforms", -1)) == int(launch_transforms):
+            return int(js["traffic_bytes_per_launch"]), os.path.relpath(f, ROOT)
+    return None, None
+
+
 def bench_mfma(torch, dev, stream, g, iters=5):
     """4096 c2c transforms of N=2^16, each scaled by 10^u, u ~ U[-3, 3):
     plain FP32-FMA plan against the split-precision MFMA mode, both timed on
@@ -252,11 +268,14 @@ def main():
     per_launch_ms = ms_a / launches
     bytes_per_launch = 16.0 * n * (batch / launches)
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
+    desc_a = fwd.describe()[0]
+    traffic, traffic_src = measured_traffic(desc_a, batch // launches)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "fft_pass_kernel<float,1024,16,col,EPI_TW4> (four-step pass A)",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "kernel": "four-step pass A: " + desc_a,
                 "avg_launch_us": round(per_launch_ms * 1e3, 2),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "traffic_source": traffic_src}
 
     extra = {"step_wall_ms": round(wall, 3), "step_hbm_frac_algorithmic": round(hbm_frac_step, 4),
              "pass_a_ms_per_direction": round(ms_a, 4), "pass_b_ms_per_direction": round(ms_b, 4),
