@@ -7,7 +7,8 @@
 #define AFGPU_ROWT_CT128 16
 #define AFGPU_ROWT_CT256 16
 #define AFGPU_ROWT_CT512 16
-#define AFGPU_ROWT_CT1024 16
+#define AFGPU_ROWT_CT1024 8
+#define AFGPU_ROWT_SV1_CT1024 16
 #define AFGPU_ROWT_CT2048 8
 #define AFGPU_ROWT_CT4096 4
 #define AFGPU_COL_CT_SMALL 16

@@ -7,6 +7,7 @@
 #define AFGPU_ROWT_CT256 8
 #define AFGPU_ROWT_CT512 8
 #define AFGPU_ROWT_CT1024 8
+#define AFGPU_ROWT_SV1_CT1024 8
 #define AFGPU_ROWT_CT2048 4
 #define AFGPU_ROWT_CT4096 2
 #define AFGPU_COL_CT_SMALL 8

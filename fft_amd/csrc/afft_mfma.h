@@ -18,6 +18,7 @@ struct MfmaParams {
   int n_col, col_tiles;
   int swap_in, swap_out;
   int in_cfast, out_cfast;  // 1: columns contiguous in memory (c fastest when moving data)
+  int in_nt, out_nt;        // streaming (non-temporal) loads / stores
   const h8* mats;
   const cx<float>* tw4_hi;  // optional four-step twiddle on the output
   const cx<float>* tw4_lo;

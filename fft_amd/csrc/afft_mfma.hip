@@ -153,25 +153,24 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
 #pragma unroll
   for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
 
-  // load: 16 x 256 complex into the planes
-#pragma unroll 4
+  // load: 16 x 256 complex, every load of the thread issued before the first
+  // LDS write so the whole tile is in flight at once
+  cx<float> v[kMCT];
+#pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    int c, j;
-    if (p.in_cfast) {
-      c = tid & 15;
-      j = (tid >> 4) + 16 * it;
-    } else {
-      c = it;
-      j = tid;
-    }
-    cx<float> v = {0.f, 0.f};
-    if (col0 + c < p.n_col) {
-      idx_t e = ib + (idx_t)(col0 + c) * p.in_cs + (idx_t)j * p.in_js;
-      v = reinterpret_cast<const cx<float>*>(p.in)[e];
-      if (p.swap_in) v = swapri(v);
-    }
-    lre[c * kMLP + j] = v.x;
-    lim[c * kMLP + j] = v.y;
+    const int c = p.in_cfast ? (tid & 15) : it;
+    const int j = p.in_cfast ? (tid >> 4) + 16 * it : tid;
+    v[it] = {0.f, 0.f};
+    if (col0 + c < p.n_col)
+      v[it] = ld_elem<float>(p.in, nullptr, 1, ib + (idx_t)(col0 + c) * p.in_cs + (idx_t)j * p.in_js, p.in_nt);
+  }
+#pragma unroll
+  for (int it = 0; it < kMCT; it++) {
+    const int c = p.in_cfast ? (tid & 15) : it;
+    const int j = p.in_cfast ? (tid >> 4) + 16 * it : tid;
+    const cx<float> z = p.swap_in ? swapri(v[it]) : v[it];
+    lre[c * kMLP + j] = z.x;
+    lim[c * kMLP + j] = z.y;
   }
   __syncthreads();
 
@@ -221,22 +220,25 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   __syncthreads();
 
   // store (optional four-step twiddle W_N^(n2 * k1), n2 = column, k1 = bin)
-#pragma unroll 4
+  const unsigned tmask = (1u << p.tw4_bits) - 1u;
+#pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    int c, k;
-    if (p.out_cfast) {
-      c = tid & 15;
-      k = (tid >> 4) + 16 * it;
-    } else {
-      c = it;
-      k = tid;
-    }
+    const int c = p.out_cfast ? (tid & 15) : it;
+    const int k = p.out_cfast ? (tid >> 4) + 16 * it : tid;
+    v[it] = {lre[c * kMLP + k], lim[c * kMLP + k]};
+  }
+#pragma unroll
+  for (int it = 0; it < kMCT; it++) {
+    const int c = p.out_cfast ? (tid & 15) : it;
+    const int k = p.out_cfast ? (tid >> 4) + 16 * it : tid;
     if (col0 + c >= p.n_col) continue;
-    cx<float> v = {lre[c * kMLP + k], lim[c * kMLP + k]};
-    if (p.tw4_hi) v = cmul(v, tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_bits, (idx_t)((col0 + c) >> p.tw4_shift) * k));
-    if (p.swap_out) v = swapri(v);
-    idx_t e = ob + (idx_t)(col0 + c) * p.out_cs + (idx_t)k * p.out_ks;
-    reinterpret_cast<cx<float>*>(p.out)[e] = v;
+    cx<float> z = v[it];
+    if (p.tw4_hi) {
+      const unsigned mi = (unsigned)((col0 + c) >> p.tw4_shift) * (unsigned)k;  // < N <= 2^24
+      z = cmul(z, tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_bits, tmask, mi));
+    }
+    if (p.swap_out) z = swapri(z);
+    st_elem<float>(p.out, nullptr, 1, ob + (idx_t)(col0 + c) * p.out_cs + (idx_t)k * p.out_ks, z, p.out_nt);
   }
 }
 

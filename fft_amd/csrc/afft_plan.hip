@@ -109,7 +109,7 @@ struct Variant {
   int stream_io = 1;  // non-temporal loads of src / stores of dst (keeps the MALL for scratch)
   int xcd_order = 0;  // XCD-contiguous tile order
   idx_t chunk = 0;    // transforms per pass group (0 = default)
-  int sched = 0;      // 1: radix-32 schedules for the long passes where compiled
+  int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
 };
 
@@ -245,24 +245,27 @@ struct Builder {
   }
 
   bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t) {
-    int CT = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
+    // tile: the variant's choice for column / transposed-store passes, else
+    // the compiled default; schedule: radix-32 where compiled unless the
+    // variant opts out; then the half-LDS kernel of that shape if one exists
+    const int deflt = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
+    const int want = colmap ? P.var.col_tile : (store_t ? P.var.rowt_tile : 0);
     const PassVariant<T>* v = nullptr;
-    const int sv = P.var.sched;
-    if (sv && colmap) v = find_pass<T>(L, P.var.col_tile > 0 ? P.var.col_tile : 0, colmap, epi, store_t, sv);
-    if (sv && store_t && !v) v = find_pass<T>(L, P.var.rowt_tile > 0 ? P.var.rowt_tile : 0, colmap, epi, store_t, sv);
-    if (!v && colmap && P.var.col_tile > 0) v = find_pass<T>(L, P.var.col_tile, colmap, epi, store_t);
-    if (!v && store_t && P.var.rowt_tile > 0) v = find_pass<T>(L, P.var.rowt_tile, colmap, epi, store_t);
-    if (v) CT = v->CT;
-    if (!v) v = find_pass<T>(L, CT, colmap, epi, store_t);
+    for (int ct : {want > 0 ? want : deflt, deflt}) {
+      if (P.var.sched >= 0) v = find_pass<T>(L, ct, colmap, epi, store_t, 1);
+      if (!v) v = find_pass<T>(L, ct, colmap, epi, store_t, 0);
+      if (v) break;
+    }
     if (!v) {
       set_err("no kernel for L=%d colmap=%d epi=%d store_t=%d", L, colmap, epi, store_t);
       ok = false;
       return false;
     }
     if (P.var.half_lds >= 0) {
-      const PassVariant<T>* h = find_pass<T>(L, CT, colmap, epi, store_t, v->sv, 1);
+      const PassVariant<T>* h = find_pass<T>(L, v->CT, colmap, epi, store_t, v->sv, 1);
       if (h) v = h;
     }
+    const int CT = v->CT;
     r->type = PT_FFT;
     r->L = L;
     r->CT = CT;
@@ -728,6 +731,8 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     m.swap_out = r.swap_out;
     m.in_cfast = r.in.s_c == 1;
     m.out_cfast = r.out.s_c == 1;
+    m.in_nt = r.in_nt;
+    m.out_nt = r.out_nt;
     m.mats = (const h8*)r.mfma_mats;
     if (r.epi == EPI_TW4) {
       m.tw4_hi = (const cx<float>*)r.tw4_hi;
@@ -1073,10 +1078,10 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   }
   // the XCD order stays a manual knob: it has not won on any measured shape
   for (int col : {0, 8})
-    for (int rowt : {0, 8})
+    for (int rowt : {0, 4, 16})
       for (int sio : {1, 0})
         for (int hl : {0, -1})
-          for (int sch : {0, 1})
+          for (int sch : {0, -1})
             for (int cm : {1, 2, 4}) {
               Variant v;
               v.sched = sch;

@@ -91,11 +91,12 @@ int afft_gpu_mfma_selftest(void);
 typedef struct
 {
   int32_t col_tile;  /* columns per work-group of column passes with L >= 512 (8 or 16) */
-  int32_t rowt_tile; /* rows per work-group of transposed-store row passes (8 or 16) */
+  int32_t rowt_tile; /* rows per work-group of transposed-store row passes (4, 8 or 16) */
   int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
   int64_t chunk;     /* transforms per pass group of four-step plans */
-  int32_t sched;     /* 1: radix-32 stage schedules for the long passes where compiled */
+  int32_t sched;     /* 1 or 0 (default): radix-32 stage schedules for the long passes
+                        where compiled, -1: never */
   int32_t half_lds;  /* 1: half-LDS kernels (real and imaginary parts exchanged in turn)
                         where compiled, -1: never, 0: default (on) */
 } afft_gpu_variant;

@@ -227,7 +227,7 @@ def test_errors_leave_state_clean(G):
 
 
 VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c)
-            for s in (0, 1) for h in (0, -1) for r in (0, 8) for c in (0, 8)]
+            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8)]
 
 
 @pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
@@ -277,5 +277,6 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
         for b in range(batch):
             yb = y[b].astype(np.float32 if kind == "ri" else np.complex64)
             assert O.rel_l2(yb, ref[b]) <= O.bound(n), (kind, n, v, b)
-    assert len(seen) >= 2 or n < (1 << 13), seen
+    if kind in ("c", "ci") and n > (1 << 14):
+        assert len(seen) >= 2, seen  # four-step plans do have alternatives
     p.close()
