@@ -17,4 +17,5 @@
 #define AFGPU_COL_CT1024 16
 #define AFGPU_COL_CT2048 8
 #define AFGPU_COL_CT4096 4
+#define AFGPU_DUAL 1
 #include "afft_variants.inc"

@@ -111,6 +111,7 @@ struct Variant {
   idx_t chunk = 0;    // transforms per pass group (0 = default)
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
+  int pipeline = 0;   // 1 / 0 (default): fused launches of pass A (group k) and pass B (group k-1), -1: never
 };
 
 struct Plan {
@@ -130,6 +131,11 @@ struct Plan {
   PassRec pass[kMaxPass];
   hipStream_t stream = nullptr;
   int device = 0;
+  // fused schedule of two-pass four-step plans over several groups: launch
+  // k runs pass A of group k and pass B of group k-1 (DualVariant); scratch
+  // 0 then holds two group buffers, used alternately
+  const void* dual = nullptr;
+  int pipe = 0;
 };
 
 struct Header {
@@ -497,7 +503,34 @@ struct Builder {
         if (!q.mfma_mats) ok = false;
       }
     }
+    if (r && ok) pick_dual();
     return r && ok;
+  }
+
+  // Two-pass four-step plans whose pass shapes have a fused kernel run as
+  // launches of (pass A of group k, pass B of group k-1); pass A then uses the
+  // fused kernel's tile shape also when it runs on its own.
+  void pick_dual() {
+    P.dual = nullptr;
+    if (P.var.pipeline < 0 || P.npass != 2 || P.mode != afft_gpu_mode_fma) return;
+    PassRec& a = P.pass[0];
+    PassRec& b = P.pass[1];
+    if (a.type != PT_FFT || b.type != PT_FFT || !a.colmap || a.epi != EPI_TW4 || a.store_t || b.colmap ||
+        !b.store_t || b.epi != EPI_PLAIN || a.in.role != R_SRC || a.out.role != R_SCR || b.in.role != R_SCR ||
+        b.out.role != R_DST || !a.in.il || !a.out.il || !b.in.il || !b.out.il || a.mfma || b.mfma)
+      return;
+    const DualVariant<T>* d = find_dual<T>(a.L, b.L, b.CT, b.sv, b.hx);
+    if (!d) return;
+    const PassVariant<T>* va = find_pass<T>(a.L, d->cta, 1, EPI_TW4, 0, d->sva, d->hxa);
+    if (!va) return;
+    const void* tw = get_table<T>(0, a.L, va->sv);
+    if (!tw) return;
+    a.variant = va;
+    a.CT = va->CT;
+    a.sv = va->sv;
+    a.hx = va->hx;
+    a.stage_tw = tw;
+    P.dual = d;
   }
 
   bool build_passes(int kind, int format, const idx_t* d, int nd) {
@@ -698,6 +731,66 @@ void resolve(const Side& s, const Bufs<T>& B, idx_t start, const T*& re, const T
   }
 }
 
+// Kernel arguments of an FFT pass over user transforms [start, start+cnt);
+// `fast` tells whether the buffer-resource path applies.
+template <typename T>
+bool fft_params(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, PassParams<T>& p, bool& fast) {
+  const T *ire, *iim, *ore_c, *oim_c;
+  resolve(r.in, B, start, ire, iim);
+  resolve(r.out, B, start, ore_c, oim_c);
+  T* ore = const_cast<T*>(ore_c);
+  T* oim = const_cast<T*>(oim_c);
+  p = PassParams<T>{};
+  p.in_re = ire;
+  p.in_im = iim;
+  p.out_re = ore;
+  p.out_im = oim;
+  p.in_js = r.in.s_j;
+  p.in_cs = r.in.s_c;
+  p.in_bs = r.in.s_b;
+  p.in_bs2 = r.in.s_b2;
+  p.out_ks = r.out.s_j;
+  p.out_cs = r.out.s_c;
+  p.out_bs = r.out.s_b;
+  p.out_bs2 = r.out.s_b2;
+  p.n_blo = r.n_blo;
+  idx_t ncol = r.fold_user ? r.n_col * cnt : r.n_col;
+  idx_t nb = r.fold_user ? 1 : r.b_per_user * cnt;
+  idx_t tiles = (ncol + r.CT - 1) / r.CT;
+  if (ncol > 0x7fffffff || tiles * nb > 0x7fffffff) {
+    set_err("launch too large");
+    return false;
+  }
+  p.n_col = (int)ncol;
+  p.col_tiles = (int)tiles;
+  p.n_tiles = (int)(tiles * nb);
+  p.in_il = r.in.il;
+  p.out_il = r.out.il;
+  p.swap_in = r.swap_in;
+  p.swap_out = r.swap_out;
+  p.in_nt = r.in_nt;
+  p.out_nt = r.out_nt;
+  p.xcd_remap = r.xcd;
+  p.stage_tw = (const cx<T>*)r.stage_tw;
+  p.tw4_hi = (const cx<T>*)r.tw4_hi;
+  p.tw4_lo = (const cx<T>*)r.tw4_lo;
+  p.tw4_lo_bits = r.tw4_bits;
+  p.tw4_col_shift = r.tw4_shift;
+  p.real_tw = (const cx<T>*)r.real_tw;
+  // byte extent touched inside one batch entry (both sides); the buffer
+  // path needs it below 4 GiB
+  const idx_t lin = (idx_t)r.L + (r.epi == EPI_C2R ? 1 : 0);
+  const idx_t lout = (idx_t)r.L + (r.epi == EPI_R2C ? 1 : 0);
+  const idx_t ext_in = ((ncol - 1) * r.in.s_c + (lin - 1) * r.in.s_j + 1) * (idx_t)sizeof(cx<T>);
+  const idx_t ext_out = ((ncol - 1) * r.out.s_c + (lout - 1) * r.out.s_j + 1) * (idx_t)sizeof(cx<T>);
+  const bool fast_ok = r.in.il && r.out.il && ext_in > 0 && ext_out > 0 && ext_in < ((idx_t)1 << 32) &&
+                       ext_out < ((idx_t)1 << 32);
+  p.in_bytes = fast_ok ? (unsigned)ext_in : 0u;
+  p.out_bytes = fast_ok ? (unsigned)ext_out : 0u;
+  fast = fast_ok;
+  return true;
+}
+
 template <typename T>
 bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStream_t st) {
   const T *ire, *iim, *ore_c, *oim_c;
@@ -745,53 +838,9 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
   }
   if (r.type == PT_FFT) {
     const PassVariant<T>* v = (const PassVariant<T>*)r.variant;
-    PassParams<T> p{};
-    p.in_re = ire;
-    p.in_im = iim;
-    p.out_re = ore;
-    p.out_im = oim;
-    p.in_js = r.in.s_j;
-    p.in_cs = r.in.s_c;
-    p.in_bs = r.in.s_b;
-    p.in_bs2 = r.in.s_b2;
-    p.out_ks = r.out.s_j;
-    p.out_cs = r.out.s_c;
-    p.out_bs = r.out.s_b;
-    p.out_bs2 = r.out.s_b2;
-    p.n_blo = r.n_blo;
-    idx_t ncol = r.fold_user ? r.n_col * cnt : r.n_col;
-    idx_t nb = r.fold_user ? 1 : r.b_per_user * cnt;
-    idx_t tiles = (ncol + r.CT - 1) / r.CT;
-    if (ncol > 0x7fffffff || tiles * nb > 0x7fffffff) {
-      set_err("launch too large");
-      return false;
-    }
-    p.n_col = (int)ncol;
-    p.col_tiles = (int)tiles;
-    p.n_tiles = (int)(tiles * nb);
-    p.in_il = r.in.il;
-    p.out_il = r.out.il;
-    p.swap_in = r.swap_in;
-    p.swap_out = r.swap_out;
-    p.in_nt = r.in_nt;
-    p.out_nt = r.out_nt;
-    p.xcd_remap = r.xcd;
-    p.stage_tw = (const cx<T>*)r.stage_tw;
-    p.tw4_hi = (const cx<T>*)r.tw4_hi;
-    p.tw4_lo = (const cx<T>*)r.tw4_lo;
-    p.tw4_lo_bits = r.tw4_bits;
-    p.tw4_col_shift = r.tw4_shift;
-    p.real_tw = (const cx<T>*)r.real_tw;
-    // byte extent touched inside one batch entry (both sides); the buffer
-    // path needs it below 4 GiB
-    const idx_t lin = (idx_t)r.L + (r.epi == EPI_C2R ? 1 : 0);
-    const idx_t lout = (idx_t)r.L + (r.epi == EPI_R2C ? 1 : 0);
-    const idx_t ext_in = ((ncol - 1) * r.in.s_c + (lin - 1) * r.in.s_j + 1) * (idx_t)sizeof(cx<T>);
-    const idx_t ext_out = ((ncol - 1) * r.out.s_c + (lout - 1) * r.out.s_j + 1) * (idx_t)sizeof(cx<T>);
-    const bool fast = r.in.il && r.out.il && ext_in > 0 && ext_out > 0 && ext_in < ((idx_t)1 << 32) &&
-                      ext_out < ((idx_t)1 << 32);
-    p.in_bytes = fast ? (unsigned)ext_in : 0u;
-    p.out_bytes = fast ? (unsigned)ext_out : 0u;
+    PassParams<T> p;
+    bool fast = false;
+    if (!fft_params<T>(r, B, start, cnt, p, fast)) return false;
     HIP_OK(v->launch(p, st, fast));
     return true;
   }
@@ -836,6 +885,41 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
   }
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+  if (P.pipe && first == 0 && last == P.npass) {
+    // launch k: pass A of group k into scratch half k%2 beside pass B of
+    // group k-1 from half (k-1)%2; the launches are ordered on `st`
+    const DualVariant<T>* d = (const DualVariant<T>*)P.dual;
+    const size_t half = P.scr_bytes[0] / 2;
+    const idx_t ngroups = (P.batch + P.chunk - 1) / P.chunk;
+    PassParams<T> pb{};
+    bool fast_b = true;
+    for (idx_t k = 0; k <= ngroups; k++) {
+      PassParams<T> pa{};
+      bool fast_a = true;
+      if (k < ngroups) {
+        const idx_t start = k * P.chunk;
+        const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
+        Bufs<T> Bk = B;
+        Bk.scr[0] = (T*)((char*)P.scr[0] + (k & 1) * half);
+        if (!fft_params<T>(P.pass[0], Bk, start, cnt, pa, fast_a)) return false;
+      }
+      if (!fast_a || !fast_b) {
+        set_err("internal: fused four-step launch needs the buffer path on both passes");
+        return false;
+      }
+      HIP_OK(d->launch(pa, pb, st));
+      // pass B of this group runs in the next launch
+      pb = PassParams<T>{};
+      if (k < ngroups) {
+        const idx_t start = k * P.chunk;
+        const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
+        Bufs<T> Bk = B;
+        Bk.scr[0] = (T*)((char*)P.scr[0] + (k & 1) * half);
+        if (!fft_params<T>(P.pass[1], Bk, start, cnt, pb, fast_b)) return false;
+      }
+    }
+    return true;
+  }
   for (idx_t start = 0; start < P.batch; start += P.chunk) {
     idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
     for (int i = first; i < last; i++)
@@ -845,9 +929,11 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
 }
 
 bool alloc_scratch(Plan& P, size_t elem_bytes) {
+  const int pipe = (P.dual && P.batch > P.chunk) ? 1 : 0;
+  P.pipe = pipe;
   for (int i = 0; i < 2; i++) {
     if (P.scr_per_user[i] == 0) continue;
-    size_t bytes = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes;
+    size_t bytes = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes * (i == 0 && pipe ? 2 : 1);
     if (bytes == P.scr_bytes[i] && P.scr[i]) continue;
     if (P.scr[i]) (void)hipFree(P.scr[i]);
     P.scr[i] = nullptr;
@@ -1010,7 +1096,7 @@ TuneCache& tcache() {
 }
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
-//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds>]
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline>]]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1022,8 +1108,8 @@ void tcache_load_locked(TuneCache& c) {
   while (fgets(line, sizeof(line), f)) {
     Variant v;
     long long ch = 0;
-    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
-                           &v.xcd_order, &ch, &v.sched, &v.half_lds);
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
+                           &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline);
     if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
@@ -1050,8 +1136,8 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io, v.xcd_order,
-          (long long)v.chunk, v.sched, v.half_lds);
+  fprintf(f, "%s %d %d %d %d %lld %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
+          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline);
   fclose(f);
 }
 
@@ -1076,10 +1162,11 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
     if (!rebuild<T>(P)) return false;
     base_chunk = P.chunk;
   }
-  // the XCD order stays a manual knob: it has not won on any measured shape
+  // the XCD order and streaming I/O stay manual knobs: neither has lost
+  // (streaming) or won (XCD order) on a measured shape
   for (int col : {0, 8})
     for (int rowt : {0, 4, 16})
-      for (int sio : {1, 0})
+      for (int pl : {0, -1})
         for (int hl : {0, -1})
           for (int sch : {0, -1})
             for (int cm : {1, 2, 4}) {
@@ -1087,7 +1174,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
               v.sched = sch;
               v.col_tile = col;
               v.rowt_tile = rowt;
-              v.stream_io = sio;
+              v.pipeline = pl;
               v.half_lds = hl;
               v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
               if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
@@ -1307,6 +1394,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.chunk = v->chunk;
   P.var.sched = v->sched;
   P.var.half_lds = v->half_lds;
+  P.var.pipeline = v->pipeline;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1355,6 +1443,8 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->chunk = (int64_t)P.chunk;
   v->sched = P.var.sched;
   v->half_lds = P.var.half_lds;
+  v->pipeline = P.var.pipeline;
+  v->pad_ = 0;
   return 0;
 }
 
@@ -1392,7 +1482,7 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   if (r.type == PT_FFT)
     snprintf(buf, len,
              "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.mfma ? "[mfma]" : "", r.L, r.CT, r.sv ? " sv1" : "",
+             r.mfma ? "[mfma]" : (P.pipe && i < 2 ? "[fused]" : ""), r.L, r.CT, r.sv ? " sv1" : "",
              r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
              r.store_t ? " store_t" : "", (long long)r.n_col,
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);

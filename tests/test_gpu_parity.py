@@ -226,8 +226,8 @@ def test_errors_leave_state_clean(G):
     assert "power of two" in _lib.last_error()
 
 
-VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c)
-            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8)]
+VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chunk=ch)
+            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8) for pl, ch in ((0, 1), (-1, 0))]
 
 
 @pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
@@ -279,4 +279,47 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
             assert O.rel_l2(yb, ref[b]) <= O.bound(n), (kind, n, v, b)
     if kind in ("c", "ci") and n > (1 << 14):
         assert len(seen) >= 2, seen  # four-step plans do have alternatives
+    p.close()
+
+
+def test_pipelined_plan_graph_replay(G):
+    """A four-step plan over several groups runs pass B of one group on the
+    plan's second stream while pass A of the next runs on the caller's; the
+    result must equal the serial schedule bit for bit, eagerly and when the
+    whole execute is captured in a HIP graph and replayed (with eager work and
+    synchronisations between replays)."""
+    import torch
+    from fft_amd.afft import Plan
+    n, batch = 1 << 20, 6
+    say("pipelined plan + graph", n, batch)
+    x = O.complex_input(41, (batch, n))
+    src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+    ref = G.empty_dev(2 * n * batch)
+    p = Plan("c", (n,), batch)
+    p.set_variant(chunk=1, pipeline=-1)
+    p(src, None, ref, None)
+    torch.cuda.synchronize()
+    p.set_variant(chunk=1, pipeline=1)
+    dst = G.empty_dev(2 * n * batch)
+    p(src, None, dst, None)
+    torch.cuda.synchronize()
+    assert torch.equal(dst, ref)
+    y = G.to_host(dst)
+    y = (y[0::2] + 1j * y[1::2]).reshape(batch, n)
+    assert O.rel_l2(y[batch - 1].astype(np.complex64), O.fft(x[batch - 1].astype(np.complex128))) <= O.bound(n)
+    out = G.empty_dev(2 * n * batch)
+    s2 = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s2):
+        p(src, None, out, None, s2)  # warm-up on the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s2):
+            p(src, None, out, None, s2)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
     p.close()
