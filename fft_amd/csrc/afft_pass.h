@@ -508,8 +508,19 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   }
 }
 
+// Minimum waves per SIMD the compiler must allow for: half-exchange tiles
+// that fit twice in a CU's 160 KiB of LDS are meant to run two groups per CU,
+// so their registers are bounded accordingly; everything else runs one.
+template <typename T, int L, int CT, int SV, bool HX>
+constexpr int pass_min_waves() {
+  constexpr int nt = PassShape<L, CT, SV>::NT;
+  constexpr int groups = (HX && 2 * PassShape<L, CT, SV>::template lds_bytes<T, HX>() <= 160 * 1024) ? 2 : 1;
+  return groups * nt / 256 > 1 ? groups * nt / 256 : 1;
+}
+
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST, int SV, bool HX>
-__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (HX ? PassShape<L, CT, SV>::NT / 128 : 1)) fft_pass_kernel(PassParams<T> p) {
+__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (pass_min_waves<T, L, CT, SV, HX>()))
+    fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
   const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
   fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST, SV, HX>(p, tile, threadIdx.x,

@@ -21,6 +21,7 @@
 // Inverse transforms swap re/im on load and store of the forward passes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -111,7 +112,7 @@ struct Variant {
   idx_t chunk = 0;    // transforms per pass group (0 = default)
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
-  int pipeline = 0;   // 1 / 0 (default): fused launches of pass A (group k) and pass B (group k-1), -1: never
+  int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
 };
 
 struct Plan {
@@ -257,19 +258,16 @@ struct Builder {
     const int deflt = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
     const int want = colmap ? P.var.col_tile : (store_t ? P.var.rowt_tile : 0);
     const PassVariant<T>* v = nullptr;
-    for (int ct : {want > 0 ? want : deflt, deflt}) {
-      if (P.var.sched >= 0) v = find_pass<T>(L, ct, colmap, epi, store_t, 1);
-      if (!v) v = find_pass<T>(L, ct, colmap, epi, store_t, 0);
-      if (v) break;
-    }
+    for (int ct : {want > 0 ? want : deflt, deflt})
+      for (int sv : {1, 0})
+        for (int hx : {1, 0}) {
+          if (v || (sv && P.var.sched < 0) || (hx && P.var.half_lds < 0)) continue;
+          v = find_pass<T>(L, ct, colmap, epi, store_t, sv, hx);
+        }
     if (!v) {
       set_err("no kernel for L=%d colmap=%d epi=%d store_t=%d", L, colmap, epi, store_t);
       ok = false;
       return false;
-    }
-    if (P.var.half_lds >= 0) {
-      const PassVariant<T>* h = find_pass<T>(L, v->CT, colmap, epi, store_t, v->sv, 1);
-      if (h) v = h;
     }
     const int CT = v->CT;
     r->type = PT_FFT;
@@ -365,7 +363,9 @@ struct Builder {
     // four-step along the axis: n = n1 * n2
     idx_t n2, n1;
     if (inner == 1) {
-      n2 = n <= (idx_t)1024 * max_col_len<T>() ? 1024 : n / max_col_len<T>();
+      // 1024-point rows up to 2^20; beyond, 2048-point rows keep the column
+      // pass at <= 2048 points, whose tiles still span 8+ columns
+      n2 = n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
       if (P.mode == afft_gpu_mode_mfma_f16 && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
       if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
         set_err("transform length %lld is too long", (long long)n);
@@ -512,7 +512,7 @@ struct Builder {
   // fused kernel's tile shape also when it runs on its own.
   void pick_dual() {
     P.dual = nullptr;
-    if (P.var.pipeline < 0 || P.npass != 2 || P.mode != afft_gpu_mode_fma) return;
+    if (P.var.pipeline <= 0 || P.npass != 2 || P.mode != afft_gpu_mode_fma) return;
     PassRec& a = P.pass[0];
     PassRec& b = P.pass[1];
     if (a.type != PT_FFT || b.type != PT_FFT || !a.colmap || a.epi != EPI_TW4 || a.store_t || b.colmap ||
@@ -1166,7 +1166,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   // (streaming) or won (XCD order) on a measured shape
   for (int col : {0, 8})
     for (int rowt : {0, 4, 16})
-      for (int pl : {0, -1})
+      for (int pl : {0, 1})
         for (int hl : {0, -1})
           for (int sch : {0, -1})
             for (int cm : {1, 2, 4}) {

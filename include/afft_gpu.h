@@ -99,10 +99,9 @@ typedef struct
                         where compiled, -1: never */
   int32_t half_lds;  /* 1: half-LDS kernels (real and imaginary parts exchanged in turn)
                         where compiled, -1: never, 0: default (on) */
-  int32_t pipeline;  /* two-pass four-step plans over several groups: 1 or 0 (default)
-                        runs pass B of a group on a second stream of the plan while
-                        pass A of the next group runs on the caller's stream (the
-                        caller's stream still orders the whole transform); -1: never */
+  int32_t pipeline;  /* two-pass four-step plans over several groups: 1 runs pass A of
+                        group k and pass B of group k-1 in one fused launch (double-
+                        buffered scratch); 0 (default) or -1: one pass per launch */
   int32_t pad_;
 } afft_gpu_variant;
 
