@@ -67,6 +67,7 @@ struct PassParams {
   const cx<T>* tw4_lo;    // W_N^j - 1, j < 2^lo_bits
   int tw4_lo_bits;
   int tw4_col_shift;      // n2 = column >> shift
+  int tw4_blo;            // 1: n2 = low batch index instead (columns of one n2 are the c index)
   const cx<T>* real_tw;   // W_{2L}^k, k < L
   // Byte extents of the in / out regions of one batch entry (fast path only).
   unsigned in_bytes, out_bytes;
@@ -431,7 +432,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       // twiddle and store four bins at a time: the scheduling barrier keeps
       // the table loads of later groups from being hoisted, which would hold
       // every twiddle in registers at once
-      const unsigned n2 = (unsigned)(col >> p.tw4_col_shift);
+      const unsigned n2 = p.tw4_blo ? (unsigned)blo : (unsigned)(col >> p.tw4_col_shift);
       const unsigned mask = (1u << p.tw4_lo_bits) - 1u;
       static_for<(E + 3) / 4>([&](auto gc) AFGPU_INLINE {
         constexpr int g = decltype(gc)::value;
@@ -447,7 +448,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       return;
     }
     if constexpr (EPI == EPI_TW4) {
-      const idx_t n2 = (idx_t)(col >> p.tw4_col_shift);
+      const idx_t n2 = p.tw4_blo ? (idx_t)blo : (idx_t)(col >> p.tw4_col_shift);
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
         const idx_t k1 = t + m * TPT;

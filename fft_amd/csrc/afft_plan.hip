@@ -94,6 +94,7 @@ struct PassRec {
   const void* tw4_hi = nullptr;
   const void* tw4_lo = nullptr;
   int tw4_bits = 0, tw4_shift = 0;
+  int tw4_blo = 0;  // four-step twiddle index from the low batch index
   const void* real_tw = nullptr;
   idx_t rr_h = 0, rr_m = 0, rr_rs = 0;
   int rr_inv = 0;
@@ -103,6 +104,11 @@ struct PassRec {
   int mfma = 0;                  // run on the split-precision MFMA DFT-256 tile
   const void* mfma_mats = nullptr;
 };
+
+// Strided (column) axes up to this length run as one column pass; longer
+// ones as a two-pass column four-step with balanced factors, whose tiles span
+// more contiguous columns than one long column tile can.
+constexpr idx_t kColDirectMax = 2048;
 
 struct Variant {
   int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
@@ -258,11 +264,19 @@ struct Builder {
     const int deflt = colmap ? col_tile<T>(L) : (store_t ? rowt_tile<T>(L) : row_tile<T>(L));
     const int want = colmap ? P.var.col_tile : (store_t ? P.var.rowt_tile : 0);
     const PassVariant<T>* v = nullptr;
-    for (int ct : {want > 0 ? want : deflt, deflt})
+    // column passes without an explicit tile first try a half-LDS kernel with
+    // twice the default tile (wider row segments at the same LDS footprint)
+    const bool wide = want <= 0 && colmap && P.var.half_lds >= 0;
+    struct Cand {
+      int ct, hx_only;
+    };
+    const Cand cands[3] = {{wide ? 2 * deflt : 0, 1}, {want > 0 ? want : deflt, 0}, {deflt, 0}};
+    for (const Cand& c : cands)
       for (int sv : {1, 0})
         for (int hx : {1, 0}) {
-          if (v || (sv && P.var.sched < 0) || (hx && P.var.half_lds < 0)) continue;
-          v = find_pass<T>(L, ct, colmap, epi, store_t, sv, hx);
+          if (v || c.ct <= 0 || (c.hx_only && !hx) || (sv && P.var.sched < 0) || (hx && P.var.half_lds < 0))
+            continue;
+          v = find_pass<T>(L, c.ct, colmap, epi, store_t, sv, hx);
         }
     if (!v) {
       set_err("no kernel for L=%d colmap=%d epi=%d store_t=%d", L, colmap, epi, store_t);
@@ -343,7 +357,7 @@ struct Builder {
       ok = false;
       return;
     }
-    if (inner > 1 && n <= max_col_len<T>()) {
+    if (inner > 1 && n <= kColDirectMax) {
       PassRec* r = add();
       if (!r || !fft_pass(r, (int)n, 1, EPI_PLAIN, 0)) return;
       r->in = in;
@@ -373,8 +387,9 @@ struct Builder {
         return;
       }
     } else {
-      n2 = max_col_len<T>();
-      if (n / n2 > max_col_len<T>()) {
+      // strided axis: balanced split, both column passes <= 4096 points
+      n2 = n >> (ilog2(n) / 2);
+      if (n2 > max_col_len<T>() || n / n2 > max_col_len<T>()) {
         set_err("axis length %lld is too long", (long long)n);
         ok = false;
         return;
@@ -471,6 +486,143 @@ struct Builder {
       cur = out;
       (void)total;
     }
+  }
+
+  // The inner c2c axes of an N-D r2c, on each of the h+1 rows of dst.
+  void inner_axes_on_dst(const Side& usr_dst, const idx_t* d, int nd, idx_t h) {
+    idx_t inner = 1;
+    for (int i = 1; i < nd; i++) inner *= d[i];
+    idx_t dd[kMaxDim];
+    dd[0] = h + 1;
+    for (int i = 1; i < nd; i++) dd[i] = d[i];
+    Side o = usr_dst;
+    o.s_u = (h + 1) * inner;
+    for (int a = nd - 1; a >= 1; a--) {
+      idx_t outer = 1, inn = 1;
+      for (int i = 0; i < a; i++) outer *= dd[i];
+      for (int i = a + 1; i < nd; i++) inn *= dd[i];
+      axis(o, o, outer, dd[a], inn, 0, 0);
+    }
+  }
+
+  // Dim 0 of an N-D r2c when h = n0/2 is too long for one column pass: the
+  // h-point c2c over the row pairs (x[2j] + i x[2j+1]) as a column four-step
+  // h = h1 * h2 (pass A: h1-point columns of rows n1*h2 + n2, batch index
+  // n2, twiddle W_h^(n2*k1); pass B: h2-point columns, output row k1 + h1*k2
+  // of dst), then the real pass over row pairs (k, h-k) in place on dst.
+  void real_dim0_fourstep_r2c(const Side& usr_src, const Side& usr_dst, idx_t n0, idx_t inner) {
+    const idx_t h = n0 / 2;
+    const idx_t h1 = (idx_t)1 << (ilog2(h) / 2), h2 = h / h1;
+    need_scratch(R_SCR, h * inner);
+    const int bits = (ilog2(h) + 1) / 2;
+    const void* tw4 = get_table<T>(1, h, bits);
+    if (!tw4) {
+      ok = false;
+      return;
+    }
+    PassRec* a = add();
+    if (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0)) return;
+    a->in = usr_src;
+    a->in.il = 0;
+    a->in.one_array = 1;
+    a->in.re_off = 0;
+    a->in.im_off = inner;
+    a->in.s_j = 2 * h2 * inner;
+    a->in.s_c = 1;
+    a->in.s_b2 = 2 * inner;
+    a->in.s_b = a->in.s_u = n0 * inner;
+    scratch_side(a->out, R_SCR);
+    a->out.s_j = h2 * inner;
+    a->out.s_c = 1;
+    a->out.s_b2 = inner;
+    a->out.s_b = h * inner;
+    a->n_col = inner;
+    a->n_blo = (int)h2;
+    a->b_per_user = h2;
+    a->tw4_hi = tw4;
+    a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
+    a->tw4_bits = bits;
+    a->tw4_blo = 1;
+    PassRec* b = add();
+    if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0)) return;
+    scratch_side(b->in, R_SCR);
+    b->in.s_j = inner;
+    b->in.s_c = 1;
+    b->in.s_b2 = h2 * inner;
+    b->in.s_b = h * inner;
+    b->out = usr_dst;
+    b->out.s_j = h1 * inner;
+    b->out.s_c = 1;
+    b->out.s_b2 = inner;
+    b->out.s_b = b->out.s_u = (h + 1) * inner;
+    b->n_col = inner;
+    b->n_blo = (int)h1;
+    b->b_per_user = h1;
+    Side rr = usr_dst;
+    rr.s_j = inner;
+    rr.s_c = 1;
+    rr.s_b = rr.s_u = (h + 1) * inner;
+    real_rows(rr, rr, h, inner, 0);
+  }
+
+  // Dim 0 of an N-D c2r with a long dim 0: `cur` holds the h+1 spectrum rows
+  // after the inner axes (plan scratch); the inverse real pass runs in place
+  // on it, then the inverse h-point c2c as a column four-step whose pass B
+  // writes row k1 + h1*k2 as the real rows 2(k1 + h1*k2), 2(k1 + h1*k2) + 1.
+  void real_dim0_fourstep_c2r(const Side& cur, const Side& usr_dst, idx_t n0, idx_t inner) {
+    const idx_t h = n0 / 2;
+    const idx_t h1 = (idx_t)1 << (ilog2(h) / 2), h2 = h / h1;
+    Side rr = cur;
+    rr.s_j = inner;
+    rr.s_c = 1;
+    rr.s_b = (h + 1) * inner;
+    real_rows(rr, rr, h, inner, 1);
+    need_scratch(R_SCR2, h * inner);
+    const int bits = (ilog2(h) + 1) / 2;
+    const void* tw4 = get_table<T>(1, h, bits);
+    if (!tw4) {
+      ok = false;
+      return;
+    }
+    // pass A: h1-point columns of rows n1*h2 + n2 (column index n2*inner + c)
+    PassRec* a = add();
+    if (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0)) return;
+    a->in = cur;
+    a->in.s_j = h2 * inner;
+    a->in.s_c = 1;
+    a->in.s_b = (h + 1) * inner;
+    scratch_side(a->out, R_SCR2);
+    a->out.s_j = h2 * inner;
+    a->out.s_c = 1;
+    a->out.s_b = h * inner;
+    a->n_col = h2 * inner;
+    a->b_per_user = 1;
+    a->swap_in = 1;
+    a->tw4_hi = tw4;
+    a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
+    a->tw4_bits = bits;
+    a->tw4_shift = ilog2(inner);
+    // pass B: h2-point columns (batch index k1), real rows out
+    PassRec* b = add();
+    if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0)) return;
+    scratch_side(b->in, R_SCR2);
+    b->in.s_j = inner;
+    b->in.s_c = 1;
+    b->in.s_b2 = h2 * inner;
+    b->in.s_b = h * inner;
+    b->out = usr_dst;
+    b->out.il = 0;
+    b->out.one_array = 1;
+    b->out.re_off = 0;
+    b->out.im_off = inner;
+    b->out.s_j = 2 * h1 * inner;
+    b->out.s_c = 1;
+    b->out.s_b2 = 2 * inner;
+    b->out.s_b = b->out.s_u = n0 * inner;
+    b->n_col = inner;
+    b->n_blo = (int)h1;
+    b->b_per_user = h1;
+    b->swap_out = 1;
   }
 
   void real_rows(Side in, Side out, idx_t h, idx_t m, int inv) {
@@ -585,9 +737,10 @@ struct Builder {
         real_rows(rin, rout, h, 1, 0);
         return ok;
       }
-      if (h > max_col_len<T>()) {
-        set_err("real transforms with dim0/2 > %d are not supported for ndim > 1", max_col_len<T>());
-        return false;
+      if (h > kColDirectMax) {
+        real_dim0_fourstep_r2c(usr_src, usr_dst, n0, inner);
+        inner_axes_on_dst(usr_dst, d, nd, h);
+        return ok;
       }
       PassRec* r = add();
       if (!r || !fft_pass(r, (int)h, 1, EPI_R2C, 0)) return false;
@@ -647,10 +800,6 @@ struct Builder {
       axis(out, out, 1, h, 1, 1, 1);
       return ok;
     }
-    if (h > max_col_len<T>()) {
-      set_err("real transforms with dim0/2 > %d are not supported for ndim > 1", max_col_len<T>());
-      return false;
-    }
     idx_t dd[kMaxDim];
     dd[0] = h + 1;
     for (int i = 1; i < nd; i++) dd[i] = d[i];
@@ -666,6 +815,10 @@ struct Builder {
       for (int i = a + 1; i < nd; i++) inn *= dd[i];
       axis(cur, scr, outer, dd[a], inn, 1, 1);
       cur = scr;
+    }
+    if (h > kColDirectMax) {
+      real_dim0_fourstep_c2r(cur, usr_dst, n0, inner);
+      return ok;
     }
     PassRec* r = add();
     if (!r || !fft_pass(r, (int)h, 1, EPI_C2R, 0)) return false;
@@ -776,6 +929,7 @@ bool fft_params(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, Pass
   p.tw4_lo = (const cx<T>*)r.tw4_lo;
   p.tw4_lo_bits = r.tw4_bits;
   p.tw4_col_shift = r.tw4_shift;
+  p.tw4_blo = r.tw4_blo;
   p.real_tw = (const cx<T>*)r.real_tw;
   // byte extent touched inside one batch entry (both sides); the buffer
   // path needs it below 4 GiB
@@ -953,8 +1107,11 @@ idx_t default_chunk(const Plan& P, size_t elem_bytes) {
   idx_t per = P.scr_per_user[0] > P.scr_per_user[1] ? P.scr_per_user[0] : P.scr_per_user[1];
   if (per == 0) return P.batch;
   if (P.var.chunk > 0) return P.var.chunk < P.batch ? P.var.chunk : P.batch;
-  // keep one pass group's scratch within ~64 MiB so it stays in the MALL
-  idx_t target = (idx_t)(64u << 20) / (idx_t)(per * elem_bytes);
+  // Four-step groups: 64 MiB of scratch when pass A has long (>= 1024-point)
+  // columns; 256 MiB when they are short, where fewer, larger launches
+  // measured faster than keeping the scratch within the MALL
+  const bool short_cols = P.npass >= 1 && P.pass[0].type == PT_FFT && P.pass[0].colmap && P.pass[0].L <= 512;
+  idx_t target = (idx_t)((short_cols ? 256u : 64u) << 20) / (idx_t)(per * elem_bytes);
   if (target < 1) target = 1;
   return target < P.batch ? target : P.batch;
 }
@@ -1562,6 +1719,7 @@ int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* di
     o.swap_out = r.swap_out;
     o.tw4_bits = r.tw4_bits;
     o.tw4_shift = r.tw4_shift;
+    o.tw4_blo = r.tw4_blo;
     o.rr_h = r.rr_h;
     o.rr_m = r.rr_m;
     o.rr_inv = r.rr_inv;

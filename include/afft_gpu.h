@@ -180,7 +180,9 @@ typedef struct
   int64_t b_per_user;
   int32_t swap_in, swap_out, tw4_bits, tw4_shift;
   int64_t rr_h, rr_m;
-  int32_t rr_inv, pad_;
+  int32_t rr_inv;
+  int32_t tw4_blo; /* 1: the four-step twiddle's column index n2 is the low batch index
+                      (b % n_blo, N = L * n_blo) instead of c >> tw4_shift */
 } afft_gpu_pass_info;
 
 typedef struct

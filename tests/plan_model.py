@@ -33,7 +33,7 @@ class PassInfo(ctypes.Structure):
                 ("fold_user", ctypes.c_int32), ("n_blo", ctypes.c_int32), ("b_per_user", ctypes.c_int64),
                 ("swap_in", ctypes.c_int32), ("swap_out", ctypes.c_int32), ("tw4_bits", ctypes.c_int32),
                 ("tw4_shift", ctypes.c_int32), ("rr_h", ctypes.c_int64), ("rr_m", ctypes.c_int64),
-                ("rr_inv", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("rr_inv", ctypes.c_int32), ("tw4_blo", ctypes.c_int32)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -169,8 +169,12 @@ def _fft_pass(p, bufs, start, cnt, pi):
         x = _swap(x)
     X = np.fft.fft(x, axis=-1)
     if p.epi == 1:  # four-step twiddle
-        n2 = (np.arange(ncol) >> p.tw4_shift).reshape(1, ncol, 1)
-        N = L * (p.n_col >> p.tw4_shift)
+        if p.tw4_blo:
+            n2 = (np.arange(nb) % p.n_blo).reshape(nb, 1, 1)
+            N = L * p.n_blo
+        else:
+            n2 = (np.arange(ncol) >> p.tw4_shift).reshape(1, ncol, 1)
+            N = L * (p.n_col >> p.tw4_shift)
         k1 = np.arange(L).reshape(1, 1, L)
         X = X * np.exp(-2j * np.pi * ((n2 * k1) % N) / N)
     if p.epi == 2:  # r2c post-pass, bins 0..L

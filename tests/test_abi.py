@@ -123,7 +123,9 @@ def model_check(kind, dims, fmt, batch, prec=32):
 
 
 SHAPES = [(1,), (2,), (4,), (8,), (32,), (64,), (4096,), (16384,), (32768,), (1 << 17,), (16, 32), (8, 8, 8),
-          (2, 4), (4, 1, 8), (8192, 4), (2, 2, 2, 2), (32, 16384), (4, 32768)]
+          (2, 4), (4, 1, 8), (8192, 4), (2, 2, 2, 2), (32, 16384), (4, 32768),
+          # long strided axes / long real dim 0: column four-step (+ real pass)
+          (8192, 8), (16384, 2), (4096, 2, 2)]
 
 
 @pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])
@@ -150,9 +152,15 @@ def test_plan_shapes_of_headline_configs():
     # 2^12 c2c: one row pass
     info, passes = M.inspect("c", (4096,), IL, 1)
     assert [(p.L, p.colmap) for p in passes] == [(4096, 0)]
-    # 8192^2 r2c: fused dim-0 column r2c, then the 8192-point rows
+    # 8192^2 r2c: dim 0 (4096 complex rows) as a 64 x 64 column four-step,
+    # the real pass over row pairs, then the 8192-point rows
     info, passes = M.inspect("r", (8192, 8192), IL, 1)
-    assert [(p.L, p.colmap, p.epi) for p in passes] == [(4096, 1, 2), (8192, 0, 0)]
+    assert [(p.type, p.L, p.colmap, p.epi) for p in passes] == [(0, 64, 1, 1), (0, 64, 1, 0), (1, 1, 0, 0),
+                                                                 (0, 8192, 0, 0)]
+    assert passes[0].tw4_blo == 1 and passes[2].rr_h == 4096 and passes[2].rr_m == 8192
+    # 2048 x 8192 r2c: dim 0 short enough for one fused column r2c pass
+    info, passes = M.inspect("r", (2048, 8192), IL, 1)
+    assert [(p.L, p.colmap, p.epi) for p in passes] == [(1024, 1, 2), (8192, 0, 0)]
     # 256^3 c2c: three single passes
     info, passes = M.inspect("c", (256, 256, 256), IL, 1)
     assert len(passes) == 3
@@ -160,6 +168,6 @@ def test_plan_shapes_of_headline_configs():
 
 def test_unsupported_shapes_report_errors():
     with pytest.raises(ValueError):
-        M.inspect("r", (65536, 2), IL, 1)
+        M.inspect("c", (1 << 26, 2), IL, 1)
     with pytest.raises(ValueError):
         M.inspect("c", (12,), IL, 1)

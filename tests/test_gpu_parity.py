@@ -86,7 +86,9 @@ def test_real_1d(G, lg):
 
 
 ND_SHAPES = [(2, 4), (4, 1, 8), (16, 32), (32, 16), (8, 8, 8), (2, 2, 2, 2), (8192, 4), (4, 32768), (64, 64, 64),
-             (2, 8192), (32, 16384)]
+             (2, 8192), (32, 16384),
+             # long strided axes and long real dim 0 (column four-step, real pass over row pairs)
+             (16384, 8), (4096, 4, 4), (8192, 64)]
 
 
 @pytest.mark.parametrize("shape", ND_SHAPES, ids=lambda s: "x".join(map(str, s)))
