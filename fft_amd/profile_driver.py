@@ -4,6 +4,12 @@ under profiles/ (the exact command sits in each summary).
 
     python -m fft_amd.profile_driver c2c_2p20 [iters]
     python -m fft_amd.profile_driver copy [iters]
+    python -m fft_amd.profile_driver queue [iters]
+
+Workloads: c2c_2p20 (64 x 2^20), c2c_2p16 (1024 x 2^16), c2c_2p12, r2c_2p22
+(16 x 2^22), c2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
+and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), copy (512
+MiB device copy, the counter calibration), queue (10,000 uneven jobs).
 """
 from __future__ import annotations
 
@@ -27,6 +33,30 @@ def main():
         y = torch.empty_like(x)
         for _ in range(iters):
             _lib.lib().afft_gpu_copy(int(x.data_ptr()), int(y.data_ptr()), x.numel() * 4, None)
+    elif work == "queue":
+        # 10,000 jobs, N = 2^k, k ~ U{6..16}, 70% c2c / 30% real, fwd/inv at random
+        from fft_amd.afft import Queue
+        rng = np.random.default_rng(5)
+        lay, off = [], 0
+        for _ in range(10000):
+            lg = int(rng.integers(6, 17))
+            real = rng.random() < 0.3
+            inv = rng.random() < 0.5
+            n = 1 << lg
+            kind = (3 if inv else 2) if real else (1 if inv else 0)
+            nin = {0: 2 * n, 1: 2 * n, 2: n, 3: 2 * (n // 2 + 1)}[kind]
+            nout = {0: 2 * n, 1: 2 * n, 2: 2 * (n // 2 + 1), 3: n}[kind]
+            so = off
+            off += (nin + 63) // 64 * 64
+            do = off
+            off += (nout + 63) // 64 * 64
+            lay.append((kind, lg, so, do))
+        buf = torch.rand(off, device=dev, generator=g)
+        base = int(buf.data_ptr())
+        q = Queue(10000, 16)
+        q.prepare(Queue.jobs([(k, lg, base + 4 * so, base + 4 * do) for (k, lg, so, do) in lay]))
+        for _ in range(iters):
+            q.launch()
     else:
         shapes = {
             "c2c_2p20": ("c", (1 << 20,), 64),
@@ -35,6 +65,8 @@ def main():
             "r2c_2p22": ("r", (1 << 22,), 16),
             "c2c_256c": ("c", (256, 256, 256), 1),
             "r2c_8192s": ("r", (8192, 8192), 1),
+            "c2c_2p16_4096": ("c", (1 << 16,), 4096),
+            "c2c_2p16_4096_mfma": ("c", (1 << 16,), 4096),
         }
         kind, dims, batch = shapes[work]
         n = int(np.prod(dims))
@@ -46,6 +78,8 @@ def main():
             hs = (dims[0] // 2 + 1) * int(np.prod(dims[1:]))
             y = torch.empty(2 * hs * batch, device=dev)
         p = Plan(kind, dims, batch)
+        if work.endswith("_mfma"):
+            p.set_mode(1)
         print("passes:", p.describe(), flush=True)
         for _ in range(iters):
             p(x, None, y, None)
