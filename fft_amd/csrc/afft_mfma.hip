@@ -7,11 +7,13 @@
 // A radix-16 stage over 16 butterflies is one real matrix product
 //     [Yr; Yi] (32 x 16) = A (32 x 32) * [Xr; Xi] (32 x 16)
 // with A = [[Re C, -Im C], [Im C, Re C]] and C the 16 x 16 complex stage
-// matrix.  Operands are FP16 split in two: x*s = hi + lo (s a power of two per
-// butterfly, chosen from the butterfly's max |x| so hi uses the full FP16
-// range), A = A_hi + A_lo; the product keeps A_hi*hi + A_hi*lo + A_lo*hi (the
-// dropped A_lo*lo term is ~2^-22 relative), accumulated in FP32 by
-// v_mfma_f32_16x16x32_f16, then scaled back by 1/s exactly.  Stage 1 uses the
+// matrix.  Operands are FP16 split in two: x*s = hi + lo, A = A_hi + A_lo; the
+// product keeps A_hi*hi + A_hi*lo + A_lo*hi (the dropped A_lo*lo term is
+// ~2^-22 relative), accumulated in FP32 by v_mfma_f32_16x16x32_f16.  s is a
+// power of two per transform, taken from the transform's max |Re|, |Im| in the
+// load phase so that max*s lies in [2^10, 2^11): stage-1 outputs then stay
+// below 16 * 2^11 = 2^15, inside FP16's range, and feed stage 2 unscaled; the
+// store multiplies by 1/s exactly.  Stage 1 uses the
 // plain DFT-16 matrix; stage 2 (Ns = 16) folds the Stockham twiddles
 // W_256^(r*k) into 16 matrices C_k = F16 * diag(W_256^(r*k)), so no twiddle
 // multiply runs on the VALU.
@@ -76,28 +78,23 @@ __global__ void gen_mfma_mats_kernel(h8* out) {
 
 
 
-// Splits the 8 values of a lane (one quarter of a butterfly's 32 reals) into
-// FP16 hi/lo after scaling by 2^sh, sh shared by the 4 lanes of the butterfly.
-__device__ __forceinline__ int block_shift(const float (&x)[8]) {
-  float mx = 0.f;
+// FP16 hi/lo split of 8 values (already scaled into FP16 range): hi = x
+// rounded to FP16, lo = the FP32 remainder x - hi (exact) rounded to FP16.
+__device__ __forceinline__ void split8(const float (&x)[8], h8& hi, h8& lo) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) mx = fmaxf(mx, fabsf(x[i]));
-  mx = fmaxf(mx, __shfl_xor(mx, 16));
-  mx = fmaxf(mx, __shfl_xor(mx, 32));
+  for (int i = 0; i < 8; i++) {
+    const _Float16 h = (_Float16)x[i];
+    hi[i] = h;
+    lo[i] = (_Float16)(x[i] - (float)h);
+  }
+}
+
+// Power-of-two exponent that puts max |value| of a transform into [2^10, 2^11).
+__device__ __forceinline__ int transform_shift(float mx) {
   if (!(mx > 0.f) || !isfinite(mx)) return 0;
   int e;
   (void)frexpf(mx, &e);  // mx in [2^(e-1), 2^e)
-  return 14 - e;         // scaled max in [2^13, 2^14)
-}
-
-__device__ __forceinline__ void split8(const float (&x)[8], int sh, h8& hi, h8& lo) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    float v = ldexpf(x[i], sh);
-    _Float16 h = (_Float16)v;
-    hi[i] = h;
-    lo[i] = (_Float16)(v - (float)h);
-  }
+  return 11 - e;
 }
 
 struct AOps {
@@ -109,11 +106,11 @@ __device__ __forceinline__ AOps load_aops(const h8* __restrict__ mats, int mat, 
           mats[mat_index(mat, 1, 1, lane)]};
 }
 
-// One group: 16 butterflies (B columns), operands already in registers.
+// One group: 16 butterflies (B columns), operands already in registers and
+// already scaled into FP16 range.
 __device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f4& re, f4& im) {
-  const int sh = block_shift(x);
   h8 bh, bl;
-  split8(x, sh, bh, bl);
+  split8(x, bh, bl);
   // small cross terms first, the large hi*hi product last
   f4 z = {0.f, 0.f, 0.f, 0.f};
   re = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a0h, bl, z, 0, 0, 0);
@@ -122,11 +119,6 @@ __device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f
   im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1h, bl, z, 0, 0, 0);
   im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1l, bh, im, 0, 0, 0);
   im = __builtin_amdgcn_mfma_f32_16x16x32_f16(A.a1h, bh, im, 0, 0, 0);
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    re[j] = ldexpf(re[j], -sh);
-    im[j] = ldexpf(im[j], -sh);
-  }
 }
 
 // 16 transforms of 256 points per work-group of 256 threads (4 waves).  The
@@ -135,6 +127,7 @@ __device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f
 __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   __shared__ float lre[kMCT * kMLP];
   __shared__ float lim[kMCT * kMLP];
+  __shared__ float smax[4][kMCT];  // per-wave partial max |x| of each transform
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -146,47 +139,69 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   const idx_t ib = (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2;
   const idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2;
 
-  // the A operands of both stages, issued before the tile load so their
-  // latency overlaps it
+  // stage-1 A operands issued before the tile load so their latency overlaps
+  // it; stage 2's are issued once the tile is in LDS (overlapping stage 1)
   const AOps A1 = load_aops(p.mats, 0, lane);
-  AOps A2[4];
-#pragma unroll
-  for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
 
-  // load: 16 x 256 complex, every load of the thread issued before the first
-  // LDS write so the whole tile is in flight at once
+  // load: 16 x 256 complex, one transform per thread and 16 of its points;
+  // columns contiguous in memory: 16 lanes read one 128-byte row segment;
+  // transforms contiguous: 16 lanes read 16 consecutive points of one
+  const int lc = p.in_cfast ? (tid & 15) : (tid >> 4);
+  const int lj0 = p.in_cfast ? (tid >> 4) : (tid & 15);
   cx<float> v[kMCT];
 #pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    const int c = p.in_cfast ? (tid & 15) : it;
-    const int j = p.in_cfast ? (tid >> 4) + 16 * it : tid;
+    const int j = lj0 + 16 * it;
     v[it] = {0.f, 0.f};
-    if (col0 + c < p.n_col)
-      v[it] = ld_elem<float>(p.in, nullptr, 1, ib + (idx_t)(col0 + c) * p.in_cs + (idx_t)j * p.in_js, p.in_nt);
+    if (col0 + lc < p.n_col)
+      v[it] = ld_elem<float>(p.in, nullptr, 1, ib + (idx_t)(col0 + lc) * p.in_cs + (idx_t)j * p.in_js, p.in_nt);
+  }
+  float mx = 0.f;
+#pragma unroll
+  for (int it = 0; it < kMCT; it++) mx = fmaxf(mx, fmaxf(fabsf(v[it].x), fabsf(v[it].y)));
+  if (p.in_cfast) {
+    // the 4 lanes of this wave that hold transform lc: lanes lc, lc+16, lc+32, lc+48
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    if (lane < 16) smax[wave][lc] = mx;
+  } else {
+    // the 16 lanes 16k..16k+15 hold transform lc
+    mx = fmaxf(mx, __shfl_xor(mx, 1));
+    mx = fmaxf(mx, __shfl_xor(mx, 2));
+    mx = fmaxf(mx, __shfl_xor(mx, 4));
+    mx = fmaxf(mx, __shfl_xor(mx, 8));
+    if ((tid & 15) < 4) smax[tid & 15][lc] = mx;
   }
 #pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    const int c = p.in_cfast ? (tid & 15) : it;
-    const int j = p.in_cfast ? (tid >> 4) + 16 * it : tid;
+    const int j = lj0 + 16 * it;
     const cx<float> z = p.swap_in ? swapri(v[it]) : v[it];
-    lre[c * kMLP + j] = z.x;
-    lim[c * kMLP + j] = z.y;
+    lre[lc * kMLP + j] = z.x;
+    lim[lc * kMLP + j] = z.y;
   }
   __syncthreads();
+  AOps A2[4];
+#pragma unroll
+  for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
+  auto shift_of = [&](int c) -> int {
+    return transform_shift(fmaxf(fmaxf(smax[0][c], smax[1][c]), fmaxf(smax[2][c], smax[3][c])));
+  };
 
   const int n = lane & 15;   // B column within the group
   const int q = lane >> 4;   // quarter of K
   const int rb = 8 * (q & 1);
   const float* plane = q >= 2 ? lim : lre;
 
-  // stage 1: wave w handles columns 4w..4w+3; group = column, B column = butterfly j
+  // stage 1: wave w handles transforms 4w..4w+3; group = transform (scaled
+  // into FP16 range on the way in), B column = butterfly j
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const int c = wave * 4 + g;
     const int j = n;
+    const float sc = ldexpf(1.f, shift_of(c));
     float x[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)];
+    for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)] * sc;
     f4 re, im;
     mfma_group(A1, x, re, im);
     // the MFMA consumed every lane's loads, so the column can be overwritten
@@ -199,8 +214,8 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   }
   __syncthreads();
 
-  // stage 2: wave w handles butterflies j = 4w..4w+3 of all 16 columns;
-  // group = j (one twiddled matrix), B column = transform c
+  // stage 2: wave w handles butterflies j = 4w..4w+3 of all 16 transforms;
+  // group = j (one twiddled matrix), B column = transform c (already scaled)
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const int j = wave * 4 + g;
@@ -219,26 +234,28 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   }
   __syncthreads();
 
-  // store (optional four-step twiddle W_N^(n2 * k1), n2 = column, k1 = bin)
+  // store, undoing the transform's scale (optional four-step twiddle
+  // W_N^(n2 * k1), n2 = column, k1 = bin)
+  const int sc_ = p.out_cfast ? (tid & 15) : (tid >> 4);
+  const int sk0 = p.out_cfast ? (tid >> 4) : (tid & 15);
+  const float inv = ldexpf(1.f, -shift_of(sc_));
   const unsigned tmask = (1u << p.tw4_bits) - 1u;
 #pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    const int c = p.out_cfast ? (tid & 15) : it;
-    const int k = p.out_cfast ? (tid >> 4) + 16 * it : tid;
-    v[it] = {lre[c * kMLP + k], lim[c * kMLP + k]};
+    const int k = sk0 + 16 * it;
+    v[it] = {lre[sc_ * kMLP + k] * inv, lim[sc_ * kMLP + k] * inv};
   }
+  if (col0 + sc_ >= p.n_col) return;
 #pragma unroll
   for (int it = 0; it < kMCT; it++) {
-    const int c = p.out_cfast ? (tid & 15) : it;
-    const int k = p.out_cfast ? (tid >> 4) + 16 * it : tid;
-    if (col0 + c >= p.n_col) continue;
+    const int k = sk0 + 16 * it;
     cx<float> z = v[it];
     if (p.tw4_hi) {
-      const unsigned mi = (unsigned)((col0 + c) >> p.tw4_shift) * (unsigned)k;  // < N <= 2^24
+      const unsigned mi = (unsigned)((col0 + sc_) >> p.tw4_shift) * (unsigned)k;  // < N <= 2^24
       z = cmul(z, tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_bits, tmask, mi));
     }
     if (p.swap_out) z = swapri(z);
-    st_elem<float>(p.out, nullptr, 1, ob + (idx_t)(col0 + c) * p.out_cs + (idx_t)k * p.out_ks, z, p.out_nt);
+    st_elem<float>(p.out, nullptr, 1, ob + (idx_t)(col0 + sc_) * p.out_cs + (idx_t)k * p.out_ks, z, p.out_nt);
   }
 }
 
