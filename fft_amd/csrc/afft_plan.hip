@@ -257,7 +257,10 @@ struct Builder {
     }
   }
 
-  bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t) {
+  // pref_ct > 0: a tile tried before the defaults (split-addressed column
+  // passes ask for wide tiles, whose 4-byte re / im accesses then still cover
+  // whole 256-byte row segments)
+  bool fft_pass(PassRec* r, int L, int colmap, int epi, int store_t, int pref_ct = 0) {
     // tile: the variant's choice for column / transposed-store passes, else
     // the compiled default; schedule: radix-32 where compiled unless the
     // variant opts out; then the half-LDS kernel of that shape if one exists
@@ -270,7 +273,8 @@ struct Builder {
     struct Cand {
       int ct, hx_only;
     };
-    const Cand cands[3] = {{wide ? 2 * deflt : 0, 1}, {want > 0 ? want : deflt, 0}, {deflt, 0}};
+    const Cand cands[4] = {
+        {want > 0 ? 0 : pref_ct, 0}, {wide ? 2 * deflt : 0, 1}, {want > 0 ? want : deflt, 0}, {deflt, 0}};
     for (const Cand& c : cands)
       for (int sv : {1, 0})
         for (int hx : {1, 0}) {
@@ -521,7 +525,7 @@ struct Builder {
       return;
     }
     PassRec* a = add();
-    if (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0)) return;
+    if (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0, 4 * col_tile<T>((int)h1))) return;
     a->in = usr_src;
     a->in.il = 0;
     a->in.one_array = 1;
@@ -604,7 +608,7 @@ struct Builder {
     a->tw4_shift = ilog2(inner);
     // pass B: h2-point columns (batch index k1), real rows out
     PassRec* b = add();
-    if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0)) return;
+    if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0, 4 * col_tile<T>((int)h2))) return;
     scratch_side(b->in, R_SCR2);
     b->in.s_j = inner;
     b->in.s_c = 1;
