@@ -268,8 +268,10 @@ struct Builder {
     const int want = colmap ? P.var.col_tile : (store_t ? P.var.rowt_tile : 0);
     const PassVariant<T>* v = nullptr;
     // column passes without an explicit tile first try a half-LDS kernel with
-    // twice the default tile (wider row segments at the same LDS footprint)
+    // twice the default tile (wider row segments at the same LDS footprint);
+    // short columns (<= 128 points) a 4x-wide tile, whose LDS is still small
     const bool wide = want <= 0 && colmap && P.var.half_lds >= 0;
+    if (pref_ct <= 0 && want <= 0 && colmap && L <= 128) pref_ct = 4 * deflt;
     struct Cand {
       int ct, hx_only;
     };
