@@ -32,7 +32,8 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_error_sums_f32", "afft_gpu_error_sums_f64", "afft_gpu_copy", "afft_gpu_execute_range",
     "afft_gpu_plan_launches", "afft_gpu_plan_inspect",
     "afft_gpu_queue_memsize", "afft_gpu_queue_create", "afft_gpu_queue_prepare", "afft_gpu_queue_launch",
-    "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy",
+    "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy", "afft_gpu_queue_status",
+    "afft_gpu_queue_set_spin_limit",
     "afft_gpu_plan_set_mode", "afft_gpu_mfma_selftest", "afft_gpu_plan_set_variant", "afft_gpu_plan_get_variant",
     "afft_gpu_plan_tune",
 ]
@@ -120,6 +121,10 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_queue_num_tasks.restype = ctypes.c_int
     L.afft_gpu_queue_destroy.argtypes = [vp]
     L.afft_gpu_queue_destroy.restype = None
+    L.afft_gpu_queue_status.argtypes = [vp]
+    L.afft_gpu_queue_status.restype = ctypes.c_int
+    L.afft_gpu_queue_set_spin_limit.argtypes = [vp, ctypes.c_longlong]
+    L.afft_gpu_queue_set_spin_limit.restype = ctypes.c_int
     L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
     L.afft_gpu_plan_set_mode.restype = ctypes.c_int
     L.afft_gpu_mfma_selftest.argtypes = []

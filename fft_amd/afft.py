@@ -270,6 +270,17 @@ class Queue:
         self.prepare(jobs)
         self.launch(stream)
 
+    def status(self) -> int:
+        """0: the last launch met every dependency; 1: a wait hit the spin limit."""
+        r = _lib.lib().afft_gpu_queue_status(self.q)
+        if r < 0:
+            raise RuntimeError(_lib.last_error())
+        return r
+
+    def set_spin_limit(self, limit: int):
+        if _lib.lib().afft_gpu_queue_set_spin_limit(self.q, int(limit)) != 0:
+            raise RuntimeError(_lib.last_error())
+
     def close(self):
         if self.q:
             _lib.lib().afft_gpu_queue_destroy(self.q)

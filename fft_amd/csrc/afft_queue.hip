@@ -10,15 +10,28 @@
 //     complex length L <= 4096 (c2c n <= 4096, real n <= 8192); the tile runs
 //     them side by side exactly like a row pass, each column with its own
 //     src / dst (PassParams::col_ptrs);
-//   - a BIG task is one longer job (complex length 8192 .. 65536) that a single
-//     work-group runs as a four-step n1 x 256 transform through a private
-//     scratch slot, with the real pre/post pass as extra sweeps.
-// Tasks are ordered longest first.  The device side is a persistent kernel:
-// every work-group pops the next task index from one atomic counter and runs
-// it until the list is empty.  No task waits on another, so the launch needs
-// no co-residency and cannot deadlock.  afft_gpu_queue_launch issues only
-// kernels (a counter-reset kernel and the persistent kernel), so it can be
-// captured in a HIP graph and replayed.
+//   - a longer job (complex length m = 8192 .. 65536) becomes a chain of tile
+//     tasks over its own scratch slot: [real pre-pass tiles (c2r)] -> four-step
+//     pass-A tiles (m/256-point columns, 16..64 columns each, W_m twiddle) ->
+//     pass-B tiles (16 rows of 256 points, columns-fastest store) -> [real
+//     post-pass tiles (r2c)].  Every tile of one step can run on a different
+//     CU; a step waits for the previous step's tiles of the same job through a
+//     per-job counter.
+// Task order: the chains are interleaved with a lag of kLag jobs between a
+// step and the next (step s of job j is listed beside step 0 of job j + s*kLag),
+// so a dependent tile is normally popped long after the tiles it waits for,
+// and the scratch of the jobs in flight stays small enough for the MALL; the
+// GROUP tasks follow, longest first.
+// Device side: a persistent kernel; every work-group pops the next task index
+// from one atomic counter.  A task only ever waits for tasks listed (hence
+// popped) before it, and a popped task is being run by a resident work-group,
+// so by induction every wait ends; the wait is also bounded by a spin limit
+// that, when hit, flags the queue (afft_gpu_queue_status) and lets the kernel
+// finish.  Step completion is published with a release add at agent scope
+// after the tile's stores, and observed with an acquire load, which makes the
+// data visible across the XCDs' L2 caches.  afft_gpu_queue_launch issues only
+// kernels (a reset kernel for the counters and the persistent kernel), so it
+// can be captured in a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,18 +51,22 @@ namespace afgpu {
 const void* queue_table(int type, idx_t n, int bits);  // afft_plan.hip (twiddle cache)
 void queue_set_err(const char* msg);
 
-enum { QT_GROUP = 0, QT_BIG = 1 };
+enum { QT_GROUP = 0, QT_RPRE = 1, QT_BIGA = 2, QT_BIGB = 3, QT_RPOST = 4 };
 constexpr int kQThreads = 256;
 constexpr int kQMaxLog2 = 16;
 constexpr int kBigN2 = 256;
+constexpr int kRealTile = 2048;  // bin pairs per real-pass tile
+constexpr int kLag = 64;         // jobs between consecutive steps of one chain
 
 struct QTask {
   int type;
-  int lg;     // log2 of the complex length (group: L, big: m)
+  int lg;     // log2 of the complex length (group: L, chain: m)
   int kind;   // afft_gpu_c / ci / r / ri
-  int count;  // jobs in a group
-  int first;  // first (src, dst) pair in the pointer table
-  int pad[3];
+  int count;  // group: jobs; chain: index of this step within the job
+  int first;  // first (src, dst) pair in the pointer table; a chain's scratch follows
+  int job;    // chain: job index (dependency counters)
+  int tile;   // chain: tile index within the step
+  int need;   // chain: tiles of the previous step to wait for (0: none)
 };
 
 struct QTables {
@@ -63,8 +80,9 @@ struct QArgs {
   const unsigned long long* ptrs;
   int* counter;
   int ntasks;
-  cx<float>* scratch;
-  idx_t scratch_per_wg;
+  int* deps;       // 4 step counters per chain job
+  int* status;     // set to 1 when a dependency wait hits the spin limit
+  long long spin_limit;
   const QTables* tab;  // device copy: indexed by log2 length at run time
 };
 
@@ -104,8 +122,9 @@ __device__ __noinline__ void q_group(const QArgs& a, const QTask& tk, int tid, c
 // Real pass over one transform of h complex bins held in `z` (in place),
 // forward (bins 0..h written) or inverse (reads bins 0..h of `s`, writes
 // z[0..h)).  Same arithmetic as real_rows_kernel with m = 1.
-__device__ void q_real_pass(const cx<float>* s, cx<float>* z, idx_t h, bool inv, const cx<float>* w_tab, int tid) {
-  for (idx_t k = tid; k <= h / 2; k += kQThreads) {
+__device__ void q_real_pass(const cx<float>* s, cx<float>* z, idx_t h, bool inv, const cx<float>* w_tab, int tid,
+                            idx_t k0, idx_t k1) {
+  for (idx_t k = k0 + tid; k < k1 && k <= h / 2; k += kQThreads) {
     if (k == 0) {
       if (inv) {
         cx<float> s0 = s[0], sh = s[h];
@@ -139,76 +158,98 @@ __device__ void q_real_pass(const cx<float>* s, cx<float>* z, idx_t h, bool inv,
 }
 
 template <int N1>
-__device__ __forceinline__ void q_big_pass_a(PassParams<float>& pa, int tid, cx<float>* lds) {
+__device__ __forceinline__ void q_pass_a_tile(PassParams<float>& pa, int tile, int tid, cx<float>* lds) {
   constexpr int CT = QGroupCT<N1>::value;
   pa.col_tiles = kBigN2 / CT;
-  for (int tile = 0; tile < pa.col_tiles; tile++) {
-    fft_tile<float, N1, CT, true, EPI_TW4, false>(pa, tile, tid, lds);
-    __syncthreads();
+  fft_tile<float, N1, CT, true, EPI_TW4, false>(pa, tile, tid, lds);
+}
+
+// Pass-A tiles of a chain job (m/256-point columns, CT of them per tile).
+int q_chain_a_tiles(int lg) {
+  switch (lg - 8) {
+    case 5: return kBigN2 / QGroupCT<32>::value;
+    case 6: return kBigN2 / QGroupCT<64>::value;
+    case 7: return kBigN2 / QGroupCT<128>::value;
+    default: return kBigN2 / QGroupCT<256>::value;
   }
 }
 
-__device__ __noinline__ void q_big(const QArgs& a, const QTask& tk, int tid, cx<float>* lds) {
-  const idx_t m = idx_t(1) << tk.lg;  // complex length of the c2c part
+// One step tile of a chain job.  Buffers: src / dst of the job, its scratch
+// (m complex); c2r runs the real pre-pass from src into dst and the c2c from
+// dst; r2c runs the real post-pass in place on dst.
+__device__ __noinline__ void q_chain(const QArgs& a, const QTask& tk, int tid, cx<float>* lds) {
+  int* dep = a.deps + 4 * (idx_t)tk.job;
+  if (tk.need > 0) {
+    if (tid == 0) {
+      long long it = 0;
+      while (__hip_atomic_load(&dep[tk.count - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < tk.need) {
+        if (++it > a.spin_limit) {
+          __hip_atomic_fetch_or(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave: no stale L1 / L2 lines
+  }
+  const idx_t m = idx_t(1) << tk.lg;
   const int n1 = (int)(m / kBigN2);
   const unsigned long long* pp = a.ptrs + 2 * (idx_t)tk.first;
   const cx<float>* src = reinterpret_cast<const cx<float>*>(pp[0]);
   cx<float>* dst = reinterpret_cast<cx<float>*>(pp[1]);
-  cx<float>* scr = a.scratch + (idx_t)blockIdx.x * a.scratch_per_wg;
+  cx<float>* scr = reinterpret_cast<cx<float>*>(pp[2]);
   const bool swap = tk.kind == afft_gpu_ci || tk.kind == afft_gpu_ri;
-  const cx<float>* a_in = src;
-  if (tk.kind == afft_gpu_ri) {
-    q_real_pass(src, dst, m, true, a.tab->real[tk.lg], tid);
-    __syncthreads();
-    a_in = dst;
+  if (tk.type == QT_RPRE || tk.type == QT_RPOST) {
+    const bool inv = tk.type == QT_RPRE;
+    q_real_pass(inv ? src : dst, dst, m, inv, a.tab->real[tk.lg], tid, (idx_t)tk.tile * kRealTile,
+                (idx_t)(tk.tile + 1) * kRealTile);
+  } else if (tk.type == QT_BIGA) {
+    // pass A: n1-point columns (stride 256) with the W_m twiddle, into scratch
+    PassParams<float> pa{};
+    pa.in_re = reinterpret_cast<const float*>(tk.kind == afft_gpu_ri ? dst : src);
+    pa.out_re = reinterpret_cast<float*>(scr);
+    pa.in_js = kBigN2;
+    pa.out_ks = kBigN2;
+    pa.in_cs = 1;
+    pa.out_cs = 1;
+    pa.n_col = kBigN2;
+    pa.n_blo = 1;
+    pa.in_il = pa.out_il = 1;
+    pa.swap_in = swap;
+    pa.tw4_hi = a.tab->tw4[tk.lg];
+    pa.tw4_lo = a.tab->tw4[tk.lg] + (m >> ((tk.lg + 1) / 2));
+    pa.tw4_lo_bits = (tk.lg + 1) / 2;
+    pa.tw4_col_shift = 0;
+    pa.stage_tw = a.tab->stage[tk.lg - 8];
+    switch (n1) {
+      case 32: q_pass_a_tile<32>(pa, tk.tile, tid, lds); break;
+      case 64: q_pass_a_tile<64>(pa, tk.tile, tid, lds); break;
+      case 128: q_pass_a_tile<128>(pa, tk.tile, tid, lds); break;
+      default: q_pass_a_tile<256>(pa, tk.tile, tid, lds); break;
+    }
+  } else {
+    // pass B: 16 rows of 256 points of the scratch, bins k1 + n1*k2 into dst
+    PassParams<float> pb{};
+    pb.in_re = reinterpret_cast<const float*>(scr);
+    pb.out_re = reinterpret_cast<float*>(dst);
+    pb.in_js = 1;
+    pb.in_cs = kBigN2;
+    pb.out_ks = n1;
+    pb.out_cs = 1;
+    pb.n_col = n1;
+    pb.n_blo = 1;
+    pb.in_il = pb.out_il = 1;
+    pb.swap_out = swap;
+    pb.stage_tw = a.tab->stage[8];
+    pb.col_tiles = n1 / 16;
+    fft_tile<float, kBigN2, 16, false, EPI_PLAIN, true>(pb, tk.tile, tid, lds);
   }
-  // pass A: n1-point columns (stride 256) with the W_m twiddle, into scratch
-  PassParams<float> pa{};
-  pa.in_re = reinterpret_cast<const float*>(a_in);
-  pa.out_re = reinterpret_cast<float*>(scr);
-  pa.in_js = kBigN2;
-  pa.out_ks = kBigN2;
-  pa.in_cs = 1;
-  pa.out_cs = 1;
-  pa.n_col = kBigN2;
-  pa.n_blo = 1;
-  pa.in_il = pa.out_il = 1;
-  pa.swap_in = swap;
-  pa.tw4_hi = a.tab->tw4[tk.lg];
-  pa.tw4_lo = a.tab->tw4[tk.lg] + (m >> ((tk.lg + 1) / 2));
-  pa.tw4_lo_bits = (tk.lg + 1) / 2;
-  pa.tw4_col_shift = 0;
-  int ln1 = tk.lg - 8;
-  pa.stage_tw = a.tab->stage[ln1];
-  switch (n1) {
-    case 32: q_big_pass_a<32>(pa, tid, lds); break;
-    case 64: q_big_pass_a<64>(pa, tid, lds); break;
-    case 128: q_big_pass_a<128>(pa, tid, lds); break;
-    default: q_big_pass_a<256>(pa, tid, lds); break;
-  }
-  // pass B: 256-point rows of the scratch, bins k1 + n1*k2 into dst
-  PassParams<float> pb{};
-  pb.in_re = reinterpret_cast<const float*>(scr);
-  pb.out_re = reinterpret_cast<float*>(dst);
-  pb.in_js = 1;
-  pb.in_cs = kBigN2;
-  pb.out_ks = n1;
-  pb.out_cs = 1;
-  pb.n_col = n1;
-  pb.n_blo = 1;
-  pb.in_il = pb.out_il = 1;
-  pb.swap_out = swap;
-  pb.stage_tw = a.tab->stage[8];
-  constexpr int CTB = 16;
-  pb.col_tiles = n1 / CTB;
-  for (int tile = 0; tile < pb.col_tiles; tile++) {
-    fft_tile<float, kBigN2, CTB, false, EPI_PLAIN, true>(pb, tile, tid, lds);
-    __syncthreads();
-  }
-  if (tk.kind == afft_gpu_r) {
-    q_real_pass(dst, dst, m, false, a.tab->real[tk.lg], tid);
-    __syncthreads();
-  }
+  // publish: every wave's stores are made visible device-wide (agent-scope
+  // release), then one add announces the tile
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) __hip_atomic_fetch_add(&dep[tk.count], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
@@ -223,8 +264,8 @@ __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
     __syncthreads();
     if (ti >= a.ntasks) return;
     const QTask tk = a.tasks[ti];
-    if (tk.type == QT_BIG) {
-      q_big(a, tk, tid, lds);
+    if (tk.type != QT_GROUP) {
+      q_chain(a, tk, tid, lds);
     } else {
       switch (tk.lg) {
         case 0: q_group<1>(a, tk, tid, lds); break;
@@ -246,8 +287,15 @@ __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
   }
 }
 
-__global__ void queue_reset_kernel(int* counter) {
-  if (threadIdx.x == 0) *counter = 0;
+// Clears the task counter, the status flag and the chain step counters (a
+// kernel, not a memset, so a captured graph replays it reliably).
+__global__ void queue_reset_kernel(int* counter, int* status, int* deps, int ndeps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    *counter = 0;
+    *status = 0;
+  }
+  for (int k = i; k < ndeps; k += gridDim.x * blockDim.x) deps[k] = 0;
 }
 
 template <int L>
@@ -292,16 +340,34 @@ struct Queue {
   int max_lg = 0;
   int grid = 0;
   int ntasks = 0;
+  int nchain = 0;                   // jobs run as tile chains
+  long long spin_limit = 1LL << 26;  // dependency-wait iterations before the queue is flagged
   QTask* d_tasks = nullptr;
+  size_t cap_tasks = 0;
   unsigned long long* d_ptrs = nullptr;
-  int* d_counter = nullptr;
+  size_t cap_ptrs = 0;
+  int* d_counter = nullptr;  // [0] task counter, [16] status
+  int* d_deps = nullptr;
+  size_t cap_deps = 0;
   cx<float>* d_scratch = nullptr;
-  idx_t scratch_per_wg = 0;
+  size_t cap_scratch = 0;  // complex elements
   QTables tab{};
   QTables* d_tab = nullptr;
   std::vector<QTask> h_tasks;
   std::vector<unsigned long long> h_ptrs;
 };
+
+// Grows a device array to at least `need` elements (contents not kept).
+template <typename X>
+hipError_t q_reserve(X*& p, size_t& cap, size_t need) {
+  if (need <= cap) return hipSuccess;
+  (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc(&p, sizeof(X) * need);
+  if (e == hipSuccess) cap = need;
+  return e;
+}
 
 struct QHeader {
   uint32_t magic;
@@ -321,6 +387,7 @@ void free_queue(Queue* q) {
   (void)hipFree(q->d_tasks);
   (void)hipFree(q->d_ptrs);
   (void)hipFree(q->d_counter);
+  (void)hipFree(q->d_deps);
   (void)hipFree(q->d_scratch);
   (void)hipFree(q->d_tab);
   delete q;
@@ -352,14 +419,8 @@ afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem)
   if (per_cu > 2) per_cu = 2;
   if (per_cu < 1) per_cu = 1;
   q->grid = ncu * per_cu;
-  // scratch: one slot per group, sized for the largest complex length in use
-  idx_t maxm = max_log2n >= 1 ? (idx_t(1) << max_log2n) : 1;
-  q->scratch_per_wg = maxm >= 8192 ? maxm : 0;
-  hipError_t e = hipMalloc(&q->d_tasks, sizeof(QTask) * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&q->d_ptrs, sizeof(unsigned long long) * 2 * max_jobs);
-  if (e == hipSuccess) e = hipMalloc(&q->d_counter, 64);
-  if (e == hipSuccess && q->scratch_per_wg)
-    e = hipMalloc(&q->d_scratch, sizeof(cx<float>) * (size_t)q->scratch_per_wg * (size_t)q->grid);
+  // task, pointer, counter and scratch arrays grow in prepare as needed
+  hipError_t e = hipMalloc(&q->d_counter, 128);
   if (e != hipSuccess) {
     queue_set_err(hipGetErrorString(e));
     free_queue(q);
@@ -427,18 +488,61 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
     auto lgm = [&](size_t i) { return jobs[i].log2n - (jobs[i].kind >= afft_gpu_r ? 1 : 0); };
     return lgm(x) > lgm(y);
   });
-  for (size_t i : big) {
-    const afft_gpu_job& j = jobs[i];
-    QTask t{};
-    t.type = QT_BIG;
-    t.kind = j.kind;
-    t.lg = j.log2n - (j.kind >= afft_gpu_r ? 1 : 0);
-    t.count = 1;
-    t.first = (int)(q->h_ptrs.size() / 2);
+  // chains: pointer pairs (src, dst) and (scratch, 0) per job, steps per job
+  struct Step {
+    int type, tiles;
+  };
+  std::vector<std::vector<Step>> steps(big.size());
+  size_t scratch_elems = 0;
+  std::vector<size_t> scratch_off(big.size());
+  for (size_t c = 0; c < big.size(); c++) {
+    const afft_gpu_job& j = jobs[big[c]];
+    const int lgm = j.log2n - (j.kind >= afft_gpu_r ? 1 : 0);
+    const int rt = (int)((((size_t)1 << lgm) / 2 + 1 + kRealTile - 1) / kRealTile);
+    if (j.kind == afft_gpu_ri) steps[c].push_back({QT_RPRE, rt});
+    steps[c].push_back({QT_BIGA, q_chain_a_tiles(lgm)});
+    steps[c].push_back({QT_BIGB, (1 << (lgm - 8)) / 16});
+    if (j.kind == afft_gpu_r) steps[c].push_back({QT_RPOST, rt});
+    scratch_off[c] = scratch_elems;
+    scratch_elems += (size_t)1 << lgm;
+  }
+  hipError_t e0 = q_reserve(q->d_scratch, q->cap_scratch, std::max<size_t>(scratch_elems, 1));
+  if (e0 == hipSuccess) e0 = q_reserve(q->d_deps, q->cap_deps, std::max<size_t>(4 * big.size(), 1));
+  if (e0 != hipSuccess) {
+    queue_set_err(hipGetErrorString(e0));
+    return -1;
+  }
+  std::vector<int> first(big.size());
+  for (size_t c = 0; c < big.size(); c++) {
+    const afft_gpu_job& j = jobs[big[c]];
+    first[c] = (int)(q->h_ptrs.size() / 2);
     q->h_ptrs.push_back(j.src);
     q->h_ptrs.push_back(j.dst);
-    q->h_tasks.push_back(t);
+    q->h_ptrs.push_back((unsigned long long)(q->d_scratch + scratch_off[c]));
+    q->h_ptrs.push_back(0);
   }
+  // step s of job c is listed at round c + s*kLag
+  const size_t nb = big.size();
+  for (size_t round = 0; nb && round < nb + 3 * (size_t)kLag; round++)
+    for (int st = 0; st < 4; st++) {
+      if (round < (size_t)st * kLag) continue;
+      const size_t c = round - (size_t)st * kLag;
+      if (c >= nb || st >= (int)steps[c].size()) continue;
+      const afft_gpu_job& j = jobs[big[c]];
+      for (int t = 0; t < steps[c][st].tiles; t++) {
+        QTask tk{};
+        tk.type = steps[c][st].type;
+        tk.kind = j.kind;
+        tk.lg = j.log2n - (j.kind >= afft_gpu_r ? 1 : 0);
+        tk.count = st;
+        tk.first = first[c];
+        tk.job = (int)c;
+        tk.tile = t;
+        tk.need = st > 0 ? steps[c][st - 1].tiles : 0;
+        q->h_tasks.push_back(tk);
+      }
+    }
+  q->nchain = (int)nb;
   for (int lgm = 12; lgm >= 0; lgm--)
     for (int kind = 0; kind < 4; kind++) {
       const auto& v = small[(size_t)kind * 13 + lgm];
@@ -458,8 +562,9 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
       }
     }
   q->ntasks = (int)q->h_tasks.size();
-  hipError_t e = hipSuccess;
-  if (q->ntasks) {
+  hipError_t e = q_reserve(q->d_tasks, q->cap_tasks, std::max<size_t>(q->h_tasks.size(), 1));
+  if (e == hipSuccess) e = q_reserve(q->d_ptrs, q->cap_ptrs, std::max<size_t>(q->h_ptrs.size(), 1));
+  if (e == hipSuccess && q->ntasks) {
     e = hipMemcpy(q->d_tasks, q->h_tasks.data(), sizeof(QTask) * q->h_tasks.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
       e = hipMemcpy(q->d_ptrs, q->h_ptrs.data(), sizeof(unsigned long long) * q->h_ptrs.size(),
@@ -485,14 +590,17 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
     (void)hipFuncSetAttribute((const void*)queue_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQLds);
     attr = true;
   }
-  hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, q->d_counter);
+  const int ndeps = 4 * q->nchain;
+  hipLaunchKernelGGL(queue_reset_kernel, dim3((unsigned)std::max(1, std::min(64, (ndeps + 255) / 256))), dim3(256),
+                     0, st, q->d_counter, q->d_counter + 16, q->d_deps, ndeps);
   QArgs a{};
   a.tasks = q->d_tasks;
   a.ptrs = q->d_ptrs;
   a.counter = q->d_counter;
   a.ntasks = q->ntasks;
-  a.scratch = q->d_scratch;
-  a.scratch_per_wg = q->scratch_per_wg;
+  a.deps = q->d_deps;
+  a.status = q->d_counter + 16;
+  a.spin_limit = q->spin_limit;
   a.tab = q->d_tab;
   int grid = std::min(q->grid, q->ntasks);
   hipLaunchKernelGGL(queue_kernel, dim3(grid), dim3(kQThreads), kQLds, st, a);
@@ -507,6 +615,30 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
 int afft_gpu_queue_run(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t njobs, void* stream) {
   if (afft_gpu_queue_prepare(qh, jobs, njobs) < 0) return -1;
   return afft_gpu_queue_launch(qh, stream);
+}
+
+int afft_gpu_queue_status(const afft_gpu_queue* qh) {
+  Queue* q = get_queue(qh);
+  if (!q) {
+    queue_set_err("queue: not a queue");
+    return -1;
+  }
+  int v = 0;
+  if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&v, q->d_counter + 16, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+    queue_set_err("queue: status read failed");
+    return -1;
+  }
+  return v;
+}
+
+int afft_gpu_queue_set_spin_limit(afft_gpu_queue* qh, long long limit) {
+  Queue* q = get_queue(qh);
+  if (!q || limit < 0) {
+    queue_set_err("queue: not a queue or negative limit");
+    return -1;
+  }
+  q->spin_limit = limit;
+  return 0;
 }
 
 int afft_gpu_queue_num_tasks(const afft_gpu_queue* qh) {

@@ -148,7 +148,10 @@ size_t afft_gpu_queue_memsize(void);
 /* A queue for up to max_jobs jobs of sizes up to 2^max_log2n (<= 2^16). */
 afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem);
 /* Groups the jobs into tasks and uploads the task list (synchronous host
- * work).  Returns the number of tasks, or -1. */
+ * work; grows the queue's device arrays and the scratch of the long jobs as
+ * needed).  Jobs of complex length <= 4096 run in groups of one tile; longer
+ * ones as chains of tiles (real pre-pass, four-step pass A, pass B, real
+ * post-pass) spread over the whole GPU.  Returns the number of tasks, or -1. */
 int afft_gpu_queue_prepare(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs);
 /* Runs the prepared task list in one persistent launch on `stream`.  Issues
  * kernels only, so it can be captured in a HIP graph and replayed. */
@@ -156,6 +159,14 @@ int afft_gpu_queue_launch(afft_gpu_queue* q, void* stream);
 /* prepare + launch. */
 int afft_gpu_queue_run(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs, void* stream);
 int afft_gpu_queue_num_tasks(const afft_gpu_queue* q);
+/* Waits for the device, then returns 0 if the last launch ran every task with
+ * its dependencies met, 1 if a dependency wait of a long job's tile hit the
+ * spin limit (the launch still completes; that job's output is then not
+ * defined), -1 on error. */
+int afft_gpu_queue_status(const afft_gpu_queue* q);
+/* Iterations a tile waits for the tiles it depends on before it flags the
+ * queue and proceeds (default 2^26, about a second). */
+int afft_gpu_queue_set_spin_limit(afft_gpu_queue* q, long long limit);
 void afft_gpu_queue_destroy(afft_gpu_queue* q);
 
 /* ---- plan inspection (host only, no GPU needed) ---------------------------- */

@@ -122,9 +122,54 @@ def test_queue_all_sizes_and_kinds(G):
     host = fill_inputs(jobs, off, 100)
     say("queue", len(jobs), "jobs")
     q, buf, recs, ntask = run_queue(G, jobs, off, host)
+    assert q.status() == 0
     out = G.to_host(buf)
     worst = check_all(jobs, host, out)
     say("queue ok, tasks", ntask, "worst err/bound", worst)
+    q.close()
+
+
+def test_queue_chains_many_long_jobs(G):
+    """Hundreds of long jobs (tile chains with dependency counters, longer
+    than the scheduling lag) of every kind, twice in a row on one queue."""
+    rng = np.random.default_rng(21)
+    jobs, total = make_jobs(rng, 240, 13, 16)
+    host = fill_inputs(jobs, total, 300)
+    say("queue chains", len(jobs))
+    q, buf, recs, ntask = run_queue(G, jobs, total, host)
+    assert q.status() == 0
+    check_all(jobs, host, G.to_host(buf))
+    import torch
+    q.launch()
+    torch.cuda.synchronize()
+    assert q.status() == 0
+    check_all(jobs, host, G.to_host(buf))
+    q.close()
+
+
+def test_queue_dependency_wait_is_bounded(G):
+    """A zero spin limit makes a tile give up waiting at once: the launch must
+    still finish (the status then reports the unmet wait, or 0 if every
+    dependency happened to be complete already)."""
+    import torch
+    from fft_amd.afft import Queue
+    rng = np.random.default_rng(3)
+    jobs, total = make_jobs(rng, 4, 16, 16)
+    host = fill_inputs(jobs, total, 11)
+    buf = G.to_dev(host)
+    base = int(buf.data_ptr())
+    q = Queue(len(jobs), 16)
+    q.set_spin_limit(0)
+    q.prepare(Queue.jobs([(k, lg, base + 4 * so, base + 4 * do) for (k, lg, so, nin, do, nout) in jobs]))
+    say("queue spin limit 0")
+    q.launch()
+    torch.cuda.synchronize()
+    assert q.status() in (0, 1)
+    q.set_spin_limit(1 << 26)
+    q.launch()
+    torch.cuda.synchronize()
+    assert q.status() == 0
+    check_all(jobs, host, G.to_host(buf))
     q.close()
 
 
