@@ -17,21 +17,28 @@
 //     post-pass tiles (r2c)].  Every tile of one step can run on a different
 //     CU; a step waits for the previous step's tiles of the same job through a
 //     per-job counter.
-// Task order: the chains are interleaved with a lag of kLag jobs between a
-// step and the next (step s of job j is listed beside step 0 of job j + s*kLag),
-// so a dependent tile is normally popped long after the tiles it waits for,
-// and the scratch of the jobs in flight stays small enough for the MALL; the
-// GROUP tasks follow, longest first.
-// Device side: a persistent kernel; every work-group pops the next task index
-// from one atomic counter.  A task only ever waits for tasks listed (hence
-// popped) before it, and a popped task is being run by a resident work-group,
-// so by induction every wait ends; the wait is also bounded by a spin limit
-// that, when hit, flags the queue (afft_gpu_queue_status) and lets the kernel
-// finish.  Step completion is published with a release add at agent scope
-// after the tile's stores, and observed with an acquire load, which makes the
-// data visible across the XCDs' L2 caches.  afft_gpu_queue_launch issues only
-// kernels (a reset kernel for the counters and the persistent kernel), so it
-// can be captured in a HIP graph and replayed.
+// Placement: each long job is assigned to one of the 8 XCDs (largest first,
+// to the least loaded), and its tiles go to that XCD's task list; a
+// work-group reads its XCD id from the hardware (HW_REG_XCC_ID) and pops its
+// own list, so every tile of a job runs under one L2.  Within a list the
+// chains are interleaved with a lag of kLag jobs between a step and the next
+// (step s of job j is listed beside step 0 of job j + s*kLag), so a dependent
+// tile is normally popped long after the tiles it waits for, and the scratch
+// of the jobs in flight stays small.  The GROUP tasks form one shared list,
+// longest first, that every work-group drains once its own list is empty.
+// Device side: a persistent kernel; lists are popped through atomic counters.
+// A task only ever waits for tasks listed (hence popped) before it in its own
+// list, and a popped task is being run by a resident work-group, so by
+// induction every wait ends; the wait is also bounded by a spin limit that,
+// when hit, flags the queue (afft_gpu_queue_status) and lets the kernel finish.
+// Coherence: producer and consumer tiles of a job share the XCD's L2 and the
+// L1 is write-through, so a step publishes its tiles with every wave's stores
+// completed (s_waitcnt) before one relaxed counter add, and a waiting tile
+// polls the counter with atomics (performed in the L2) and then invalidates
+// its CU's L1 (buffer_inv sc0); no L2-wide write-back or invalidation is
+// needed.  afft_gpu_queue_launch issues only kernels (a
+// reset kernel for the counters and the persistent kernel), so it can be
+// captured in a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,7 +63,9 @@ constexpr int kQThreads = 256;
 constexpr int kQMaxLog2 = 16;
 constexpr int kBigN2 = 256;
 constexpr int kRealTile = 2048;  // bin pairs per real-pass tile
-constexpr int kLag = 64;         // jobs between consecutive steps of one chain
+constexpr int kLag = 8;          // jobs between consecutive steps of one chain (per XCD list)
+constexpr int kXcd = 8;          // XCDs of an MI355X: one chain list each
+constexpr int kCtr = 16;         // ints between counters (one 64-byte line each)
 
 struct QTask {
   int type;
@@ -76,10 +85,10 @@ struct QTables {
 };
 
 struct QArgs {
-  const QTask* tasks;
+  const QTask* tasks;  // kXcd chain lists, then the shared group list
   const unsigned long long* ptrs;
-  int* counter;
-  int ntasks;
+  int* counter;        // list l pops through counter[l * kCtr]
+  int list_off[kXcd + 2];
   int* deps;       // 4 step counters per chain job
   int* status;     // set to 1 when a dependency wait hits the spin limit
   long long spin_limit;
@@ -182,7 +191,8 @@ __device__ __noinline__ void q_chain(const QArgs& a, const QTask& tk, int tid, c
   if (tk.need > 0) {
     if (tid == 0) {
       long long it = 0;
-      while (__hip_atomic_load(&dep[tk.count - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < tk.need) {
+      // polled with an atomic (performed in the L2, never served by a stale L1 line)
+      while (__hip_atomic_fetch_add(&dep[tk.count - 1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < tk.need) {
         if (++it > a.spin_limit) {
           __hip_atomic_fetch_or(a.status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
@@ -191,7 +201,9 @@ __device__ __noinline__ void q_chain(const QArgs& a, const QTask& tk, int tid, c
       }
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave: no stale L1 / L2 lines
+    // drop this CU's L1 copies: a line shared with another job's buffer may
+    // have been read here before this job's producer tiles wrote it
+    asm volatile("buffer_inv sc0" ::: "memory");
   }
   const idx_t m = idx_t(1) << tk.lg;
   const int n1 = (int)(m / kBigN2);
@@ -245,11 +257,35 @@ __device__ __noinline__ void q_chain(const QArgs& a, const QTask& tk, int tid, c
     pb.col_tiles = n1 / 16;
     fft_tile<float, kBigN2, 16, false, EPI_PLAIN, true>(pb, tk.tile, tid, lds);
   }
-  // publish: every wave's stores are made visible device-wide (agent-scope
-  // release), then one add announces the tile
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // publish: every wave's stores have reached the L2 (s_waitcnt 0), then one
+  // add announces the tile
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (tid == 0) __hip_atomic_fetch_add(&dep[tk.count], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) __hip_atomic_fetch_add(&dep[tk.count], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int xcc_id() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return (int)(v & (kXcd - 1));
+}
+
+__device__ __noinline__ void q_group_any(const QArgs& a, const QTask& tk, int tid, cx<float>* lds) {
+  switch (tk.lg) {
+    case 0: q_group<1>(a, tk, tid, lds); break;
+    case 1: q_group<2>(a, tk, tid, lds); break;
+    case 2: q_group<4>(a, tk, tid, lds); break;
+    case 3: q_group<8>(a, tk, tid, lds); break;
+    case 4: q_group<16>(a, tk, tid, lds); break;
+    case 5: q_group<32>(a, tk, tid, lds); break;
+    case 6: q_group<64>(a, tk, tid, lds); break;
+    case 7: q_group<128>(a, tk, tid, lds); break;
+    case 8: q_group<256>(a, tk, tid, lds); break;
+    case 9: q_group<512>(a, tk, tid, lds); break;
+    case 10: q_group<1024>(a, tk, tid, lds); break;
+    case 11: q_group<2048>(a, tk, tid, lds); break;
+    default: q_group<4096>(a, tk, tid, lds); break;
+  }
 }
 
 __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
@@ -257,33 +293,24 @@ __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
   cx<float>* lds = reinterpret_cast<cx<float>*>(q_smem);
   __shared__ int s_task;
   const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_task = atomicAdd(a.counter, 1);
-    __syncthreads();
-    const int ti = s_task;
-    __syncthreads();
-    if (ti >= a.ntasks) return;
-    const QTask tk = a.tasks[ti];
-    if (tk.type != QT_GROUP) {
-      q_chain(a, tk, tid, lds);
-    } else {
-      switch (tk.lg) {
-        case 0: q_group<1>(a, tk, tid, lds); break;
-        case 1: q_group<2>(a, tk, tid, lds); break;
-        case 2: q_group<4>(a, tk, tid, lds); break;
-        case 3: q_group<8>(a, tk, tid, lds); break;
-        case 4: q_group<16>(a, tk, tid, lds); break;
-        case 5: q_group<32>(a, tk, tid, lds); break;
-        case 6: q_group<64>(a, tk, tid, lds); break;
-        case 7: q_group<128>(a, tk, tid, lds); break;
-        case 8: q_group<256>(a, tk, tid, lds); break;
-        case 9: q_group<512>(a, tk, tid, lds); break;
-        case 10: q_group<1024>(a, tk, tid, lds); break;
-        case 11: q_group<2048>(a, tk, tid, lds); break;
-        default: q_group<4096>(a, tk, tid, lds); break;
-      }
+  // this XCD's chain list first, then the shared group list
+  const int lists[2] = {xcc_id(), kXcd};
+  for (int li = 0; li < 2; li++) {
+    const int l = lists[li];
+    const int n = a.list_off[l + 1] - a.list_off[l];
+    for (;;) {
+      if (tid == 0) s_task = atomicAdd(a.counter + l * kCtr, 1);
+      __syncthreads();
+      const int ti = s_task;
+      __syncthreads();
+      if (ti >= n) break;
+      const QTask tk = a.tasks[a.list_off[l] + ti];
+      if (tk.type != QT_GROUP)
+        q_chain(a, tk, tid, lds);
+      else
+        q_group_any(a, tk, tid, lds);
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -291,10 +318,8 @@ __global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
 // kernel, not a memset, so a captured graph replays it reliably).
 __global__ void queue_reset_kernel(int* counter, int* status, int* deps, int ndeps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    *counter = 0;
-    *status = 0;
-  }
+  if (i <= kXcd) counter[i * kCtr] = 0;
+  if (i == 0) *status = 0;
   for (int k = i; k < ndeps; k += gridDim.x * blockDim.x) deps[k] = 0;
 }
 
@@ -346,7 +371,8 @@ struct Queue {
   size_t cap_tasks = 0;
   unsigned long long* d_ptrs = nullptr;
   size_t cap_ptrs = 0;
-  int* d_counter = nullptr;  // [0] task counter, [16] status
+  int* d_counter = nullptr;  // list counters at [l * kCtr], l = 0..kXcd; status at [(kXcd + 1) * kCtr]
+  int list_off[kXcd + 2] = {};
   int* d_deps = nullptr;
   size_t cap_deps = 0;
   cx<float>* d_scratch = nullptr;
@@ -420,7 +446,7 @@ afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem)
   if (per_cu < 1) per_cu = 1;
   q->grid = ncu * per_cu;
   // task, pointer, counter and scratch arrays grow in prepare as needed
-  hipError_t e = hipMalloc(&q->d_counter, 128);
+  hipError_t e = hipMalloc(&q->d_counter, sizeof(int) * kCtr * (kXcd + 2));
   if (e != hipSuccess) {
     queue_set_err(hipGetErrorString(e));
     free_queue(q);
@@ -521,27 +547,44 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
     q->h_ptrs.push_back((unsigned long long)(q->d_scratch + scratch_off[c]));
     q->h_ptrs.push_back(0);
   }
-  // step s of job c is listed at round c + s*kLag
+  // jobs to XCDs: largest first (big is sorted), each to the least loaded
   const size_t nb = big.size();
-  for (size_t round = 0; nb && round < nb + 3 * (size_t)kLag; round++)
-    for (int st = 0; st < 4; st++) {
-      if (round < (size_t)st * kLag) continue;
-      const size_t c = round - (size_t)st * kLag;
-      if (c >= nb || st >= (int)steps[c].size()) continue;
-      const afft_gpu_job& j = jobs[big[c]];
-      for (int t = 0; t < steps[c][st].tiles; t++) {
-        QTask tk{};
-        tk.type = steps[c][st].type;
-        tk.kind = j.kind;
-        tk.lg = j.log2n - (j.kind >= afft_gpu_r ? 1 : 0);
-        tk.count = st;
-        tk.first = first[c];
-        tk.job = (int)c;
-        tk.tile = t;
-        tk.need = st > 0 ? steps[c][st - 1].tiles : 0;
-        q->h_tasks.push_back(tk);
-      }
+  std::vector<std::vector<size_t>> on_xcd(kXcd);
+  {
+    std::vector<double> load(kXcd, 0.0);
+    for (size_t c = 0; c < nb; c++) {
+      const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      on_xcd[x].push_back(c);
+      load[x] += (double)((size_t)1 << (jobs[big[c]].log2n));
     }
+  }
+  // per XCD list: step s of its k-th job is listed at round k + s*kLag
+  for (int x = 0; x < kXcd; x++) {
+    q->list_off[x] = (int)q->h_tasks.size();
+    const auto& js = on_xcd[x];
+    for (size_t round = 0; !js.empty() && round < js.size() + 3 * (size_t)kLag; round++)
+      for (int st = 0; st < 4; st++) {
+        if (round < (size_t)st * kLag) continue;
+        const size_t k = round - (size_t)st * kLag;
+        if (k >= js.size()) continue;
+        const size_t c = js[k];
+        if (st >= (int)steps[c].size()) continue;
+        const afft_gpu_job& j = jobs[big[c]];
+        for (int t = 0; t < steps[c][st].tiles; t++) {
+          QTask tk{};
+          tk.type = steps[c][st].type;
+          tk.kind = j.kind;
+          tk.lg = j.log2n - (j.kind >= afft_gpu_r ? 1 : 0);
+          tk.count = st;
+          tk.first = first[c];
+          tk.job = (int)c;
+          tk.tile = t;
+          tk.need = st > 0 ? steps[c][st - 1].tiles : 0;
+          q->h_tasks.push_back(tk);
+        }
+      }
+  }
+  q->list_off[kXcd] = (int)q->h_tasks.size();
   q->nchain = (int)nb;
   for (int lgm = 12; lgm >= 0; lgm--)
     for (int kind = 0; kind < 4; kind++) {
@@ -562,6 +605,7 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
       }
     }
   q->ntasks = (int)q->h_tasks.size();
+  q->list_off[kXcd + 1] = q->ntasks;
   hipError_t e = q_reserve(q->d_tasks, q->cap_tasks, std::max<size_t>(q->h_tasks.size(), 1));
   if (e == hipSuccess) e = q_reserve(q->d_ptrs, q->cap_ptrs, std::max<size_t>(q->h_ptrs.size(), 1));
   if (e == hipSuccess && q->ntasks) {
@@ -592,17 +636,18 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
   }
   const int ndeps = 4 * q->nchain;
   hipLaunchKernelGGL(queue_reset_kernel, dim3((unsigned)std::max(1, std::min(64, (ndeps + 255) / 256))), dim3(256),
-                     0, st, q->d_counter, q->d_counter + 16, q->d_deps, ndeps);
+                     0, st, q->d_counter, q->d_counter + (kXcd + 1) * kCtr, q->d_deps, ndeps);
   QArgs a{};
   a.tasks = q->d_tasks;
   a.ptrs = q->d_ptrs;
   a.counter = q->d_counter;
-  a.ntasks = q->ntasks;
+  for (int l = 0; l < kXcd + 2; l++) a.list_off[l] = q->list_off[l];
   a.deps = q->d_deps;
-  a.status = q->d_counter + 16;
+  a.status = q->d_counter + (kXcd + 1) * kCtr;
   a.spin_limit = q->spin_limit;
   a.tab = q->d_tab;
-  int grid = std::min(q->grid, q->ntasks);
+  // the full grid even for few tasks: every XCD needs work-groups for its list
+  const int grid = q->grid;
   hipLaunchKernelGGL(queue_kernel, dim3(grid), dim3(kQThreads), kQLds, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -624,7 +669,7 @@ int afft_gpu_queue_status(const afft_gpu_queue* qh) {
     return -1;
   }
   int v = 0;
-  if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&v, q->d_counter + 16, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+  if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&v, q->d_counter + (kXcd + 1) * kCtr, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
     queue_set_err("queue: status read failed");
     return -1;
   }
