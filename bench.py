@@ -193,7 +193,14 @@ def bench_queue(torch, dev, stream, reps=40):
     t0 = time.perf_counter()
     ntask = q.prepare(jobs)
     prep_ms = (time.perf_counter() - t0) * 1e3
-    q.launch(stream)
+    # persistent work-groups per CU: the faster of 2 and 4 on this batch
+    per_cu = {}
+    for groups in (2, 4):
+        q.set_groups_per_cu(groups)
+        q.launch(stream)
+        per_cu[groups] = time_events(torch, stream, lambda: q.launch(stream), 10)
+    best_groups = min(per_cu, key=per_cu.get)
+    q.set_groups_per_cu(best_groups)
     ms = time_events(torch, stream, lambda: q.launch(stream), 10)
     algb = sum(4 * (nin + nout) for (_, _, _, _, nin, nout) in lay)
     fl = sum((5.0 if k < 2 else 2.5) * (1 << lg) * lg for (k, lg, _, _, _, _) in lay)
@@ -220,6 +227,7 @@ def bench_queue(torch, dev, stream, reps=40):
     torch.cuda.empty_cache()
     return {"workload": "10000 x 1-D FP32 jobs, N=2^k k~U{6..16}, 70% c2c / 30% real, fwd/inv at random",
             "tasks": int(ntask), "prepare_host_ms": round(prep_ms, 3), "launch_ms": round(ms, 4),
+            "groups_per_cu": best_groups, "launch_ms_by_groups_per_cu": {k: round(v, 4) for k, v in per_cu.items()},
             "GFLOPs": round(fl / (ms * 1e-3) / 1e9, 1), "algorithmic_GBps": round(algb / (ms * 1e-3) / 1e9, 1),
             "graph_replay_p50_ms": round(float(np.percentile(lat, 50)), 4),
             "graph_replay_p99_ms": round(float(np.percentile(lat, 99)), 4), "graph_replays": reps}

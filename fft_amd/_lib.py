@@ -33,7 +33,7 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_plan_launches", "afft_gpu_plan_inspect",
     "afft_gpu_queue_memsize", "afft_gpu_queue_create", "afft_gpu_queue_prepare", "afft_gpu_queue_launch",
     "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy", "afft_gpu_queue_status",
-    "afft_gpu_queue_set_spin_limit",
+    "afft_gpu_queue_set_spin_limit", "afft_gpu_queue_set_groups_per_cu",
     "afft_gpu_plan_set_mode", "afft_gpu_mfma_selftest", "afft_gpu_plan_set_variant", "afft_gpu_plan_get_variant",
     "afft_gpu_plan_tune",
 ]
@@ -125,6 +125,8 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_queue_status.restype = ctypes.c_int
     L.afft_gpu_queue_set_spin_limit.argtypes = [vp, ctypes.c_longlong]
     L.afft_gpu_queue_set_spin_limit.restype = ctypes.c_int
+    L.afft_gpu_queue_set_groups_per_cu.argtypes = [vp, ctypes.c_int]
+    L.afft_gpu_queue_set_groups_per_cu.restype = ctypes.c_int
     L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
     L.afft_gpu_plan_set_mode.restype = ctypes.c_int
     L.afft_gpu_mfma_selftest.argtypes = []

@@ -277,6 +277,10 @@ class Queue:
             raise RuntimeError(_lib.last_error())
         return r
 
+    def set_groups_per_cu(self, groups: int):
+        if _lib.lib().afft_gpu_queue_set_groups_per_cu(self.q, int(groups)) != 0:
+            raise RuntimeError(_lib.last_error())
+
     def set_spin_limit(self, limit: int):
         if _lib.lib().afft_gpu_queue_set_spin_limit(self.q, int(limit)) != 0:
             raise RuntimeError(_lib.last_error())

@@ -383,9 +383,9 @@ struct Builder {
     // four-step along the axis: n = n1 * n2
     idx_t n2, n1;
     if (inner == 1) {
-      // 1024-point rows up to 2^20; beyond, 2048-point rows keep the column
-      // pass at <= 2048 points, whose tiles still span 8+ columns
-      n2 = n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
+      // 1024-point rows up to 2^21; beyond, 2048-point rows keep the column
+      // pass at <= 2048 points, whose tiles still span 16 columns
+      n2 = n <= ((idx_t)1 << 21) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
       if (P.mode == afft_gpu_mode_mfma_f16 && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
       if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
         set_err("transform length %lld is too long", (long long)n);

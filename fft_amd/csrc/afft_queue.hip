@@ -288,7 +288,10 @@ __device__ __noinline__ void q_group_any(const QArgs& a, const QTask& tk, int ti
   }
 }
 
-__global__ void __launch_bounds__(kQThreads, 2) queue_kernel(QArgs a) {
+// GROUPS = work-groups per CU the registers are bounded for (2: 256 VGPRs per
+// lane, 4: 128; the queue picks one, see afft_gpu_queue_set_groups_per_cu).
+template <int GROUPS>
+__global__ void __launch_bounds__(kQThreads, GROUPS) queue_kernel(QArgs a) {
   extern __shared__ __align__(16) unsigned char q_smem[];
   cx<float>* lds = reinterpret_cast<cx<float>*>(q_smem);
   __shared__ int s_task;
@@ -366,6 +369,8 @@ struct Queue {
   int grid = 0;
   int ntasks = 0;
   int nchain = 0;                   // jobs run as tile chains
+  int groups_per_cu = 2;            // persistent work-groups per CU (2 or 4)
+  int ncu = 256;
   long long spin_limit = 1LL << 26;  // dependency-wait iterations before the queue is flagged
   QTask* d_tasks = nullptr;
   size_t cap_tasks = 0;
@@ -439,12 +444,9 @@ afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   }
-  // persistent grid: two 256-thread groups per CU (the kernel is compiled for
-  // two waves per SIMD: 256 VGPRs, no spills)
-  int per_cu = (int)(160 * 1024 / kQLds);
-  if (per_cu > 2) per_cu = 2;
-  if (per_cu < 1) per_cu = 1;
-  q->grid = ncu * per_cu;
+  // persistent grid: groups_per_cu 256-thread groups per CU
+  q->ncu = ncu;
+  q->grid = ncu * q->groups_per_cu;
   // task, pointer, counter and scratch arrays grow in prepare as needed
   hipError_t e = hipMalloc(&q->d_counter, sizeof(int) * kCtr * (kXcd + 2));
   if (e != hipSuccess) {
@@ -629,11 +631,7 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
   }
   if (q->ntasks == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  static bool attr = false;
-  if (!attr && kQLds > 65536) {
-    (void)hipFuncSetAttribute((const void*)queue_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQLds);
-    attr = true;
-  }
+  static_assert(kQLds * 4 <= 160 * 1024, "four groups per CU must fit the LDS");
   const int ndeps = 4 * q->nchain;
   hipLaunchKernelGGL(queue_reset_kernel, dim3((unsigned)std::max(1, std::min(64, (ndeps + 255) / 256))), dim3(256),
                      0, st, q->d_counter, q->d_counter + (kXcd + 1) * kCtr, q->d_deps, ndeps);
@@ -648,7 +646,10 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
   a.tab = q->d_tab;
   // the full grid even for few tasks: every XCD needs work-groups for its list
   const int grid = q->grid;
-  hipLaunchKernelGGL(queue_kernel, dim3(grid), dim3(kQThreads), kQLds, st, a);
+  if (q->groups_per_cu == 4)
+    hipLaunchKernelGGL(queue_kernel<4>, dim3(grid), dim3(kQThreads), kQLds, st, a);
+  else
+    hipLaunchKernelGGL(queue_kernel<2>, dim3(grid), dim3(kQThreads), kQLds, st, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     queue_set_err(hipGetErrorString(e));
@@ -683,6 +684,17 @@ int afft_gpu_queue_set_spin_limit(afft_gpu_queue* qh, long long limit) {
     return -1;
   }
   q->spin_limit = limit;
+  return 0;
+}
+
+int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* qh, int groups) {
+  Queue* q = get_queue(qh);
+  if (!q || (groups != 2 && groups != 4)) {
+    queue_set_err("queue: not a queue or groups per CU not 2 or 4");
+    return -1;
+  }
+  q->groups_per_cu = groups;
+  q->grid = q->ncu * groups;
   return 0;
 }
 

@@ -167,6 +167,9 @@ int afft_gpu_queue_status(const afft_gpu_queue* q);
 /* Iterations a tile waits for the tiles it depends on before it flags the
  * queue and proceeds (default 2^26, about a second). */
 int afft_gpu_queue_set_spin_limit(afft_gpu_queue* q, long long limit);
+/* Persistent work-groups per CU: 2 (default; registers for the largest
+ * tiles) or 4 (twice the tiles in flight, registers bounded to 128). */
+int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* q, int groups);
 void afft_gpu_queue_destroy(afft_gpu_queue* q);
 
 /* ---- plan inspection (host only, no GPU needed) ---------------------------- */

@@ -131,7 +131,8 @@ def test_queue_all_sizes_and_kinds(G):
 
 def test_queue_chains_many_long_jobs(G):
     """Hundreds of long jobs (tile chains with dependency counters, longer
-    than the scheduling lag) of every kind, twice in a row on one queue."""
+    than the scheduling lag) of every kind, twice in a row on one queue, with
+    two and with four persistent work-groups per CU."""
     rng = np.random.default_rng(21)
     jobs, total = make_jobs(rng, 240, 13, 16)
     host = fill_inputs(jobs, total, 300)
@@ -140,10 +141,14 @@ def test_queue_chains_many_long_jobs(G):
     assert q.status() == 0
     check_all(jobs, host, G.to_host(buf))
     import torch
-    q.launch()
-    torch.cuda.synchronize()
-    assert q.status() == 0
-    check_all(jobs, host, G.to_host(buf))
+    for groups in (4, 2):
+        say("queue chains, groups per CU", groups)
+        q.set_groups_per_cu(groups)
+        buf.copy_(G.to_dev(host))
+        q.launch()
+        torch.cuda.synchronize()
+        assert q.status() == 0
+        check_all(jobs, host, G.to_host(buf))
     q.close()
 
 
