@@ -147,21 +147,25 @@ def bench_mfma(torch, dev, stream, g, iters=5):
     pf = Plan("c", (n,), batch)
     pm = Plan("c", (n,), batch)
     pm.set_mode(1)
+    pb = Plan("c", (n,), batch)
+    pb.set_mode(2)
     out = {}
     bound = 2.0 * 16 * 2.0 ** -24
-    for tag, p in (("fp32", pf), ("mfma", pm)):
+    for tag, p in (("fp32", pf), ("mfma", pm), ("mfma_bf16", pb)):
         p(x32, None, y, None, stream)
         ms = time_events(torch, stream, lambda: p(x32, None, y, None, stream), iters)
         d = (y.double() - ref).pow_(2).sum(1)
         r = ref.pow(2).sum(1)
         err = torch.sqrt(d / r)
         out[tag] = (ms, float(err.max()), float(err.mean()))
-    del y, ref, x32, pf, pm, p64
+    del y, ref, x32, pf, pm, pb, p64
     torch.cuda.empty_cache()
-    (mf, ef, af), (mm, em, am) = out["fp32"], out["mfma"]
+    (mf, ef, af), (mm, em, am), (mb, eb, ab) = out["fp32"], out["mfma"], out["mfma_bf16"]
     return {"workload": "4096 x c2c N=2^16 FP32 interleaved, amplitudes 10^u, u~U[-3,3) per transform",
             "fp32_fma_ms": round(mf, 4), "mfma_ms": round(mm, 4), "mfma_speedup": round(mf / mm, 3),
             "fp32_err_max": ef, "fp32_err_mean": af, "mfma_err_max": em, "mfma_err_mean": am,
+            "mfma_bf16_ms": round(mb, 4), "mfma_bf16_speedup": round(mf / mb, 3), "mfma_bf16_err_max": eb,
+            "mfma_bf16_err_mean": ab,
             "err_bound": bound, "mfma_GFLOPs": round(flops_c2c(n, batch) / (mm * 1e-3) / 1e9, 1),
             "fp32_GFLOPs": round(flops_c2c(n, batch) / (mf * 1e-3) / 1e9, 1)}
 

@@ -120,6 +120,10 @@ def align_size(n: int, dtype=np.float32) -> int:
     return int(getattr(_lib.lib(), f"afft_{p}_align_size")(int(n)))
 
 
+# afft_gpu_mode_* (include/afft_gpu.h)
+MODE_FMA, MODE_MFMA_F16, MODE_MFMA_BF16 = 0, 1, 2
+
+
 class Variant(ctypes.Structure):
     """afft_gpu_variant (include/afft_gpu.h): kernel-shape choices of a plan;
     0 = default for every field."""

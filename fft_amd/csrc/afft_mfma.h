@@ -8,6 +8,7 @@
 namespace afgpu {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 
 struct MfmaParams {
   const float* in;  // interleaved complex
@@ -19,6 +20,7 @@ struct MfmaParams {
   int swap_in, swap_out;
   int in_cfast, out_cfast;  // 1: columns contiguous in memory (c fastest when moving data)
   int in_nt, out_nt;        // streaming (non-temporal) loads / stores
+  int bf16;                 // 1: BF16 three-way split operands (table from gen_mfma_mats(.., 1, ..))
   const h8* mats;
   const cx<float>* tw4_hi;  // optional four-step twiddle on the output
   const cx<float>* tw4_lo;
@@ -26,8 +28,8 @@ struct MfmaParams {
   int tw4_shift;
 };
 
-hipError_t gen_mfma_mats(void* out, hipStream_t st);
-size_t mfma_mats_bytes();
+hipError_t gen_mfma_mats(void* out, int bf16, hipStream_t st);
+size_t mfma_mats_bytes(int bf16);
 hipError_t launch_mfma_dft256(const MfmaParams& p, int n_tiles, hipStream_t st);
 
 }  // namespace afgpu

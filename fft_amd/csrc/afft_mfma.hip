@@ -18,7 +18,14 @@
 // W_256^(r*k) into 16 matrices C_k = F16 * diag(W_256^(r*k)), so no twiddle
 // multiply runs on the VALU.
 //
-// Operand layouts (v_mfma_f32_16x16x32_f16, wave64): lane l holds row l%16 of
+// BF16 variant (afft_gpu_mode_mfma_bf16, v_mfma_f32_16x16x32_bf16): BF16 keeps
+// 8 significand bits, so operands are split three ways, x = hi + mid + lo and
+// A = A_hi + A_mid + A_lo, and the six products whose order is at least 2^-16
+// are kept (hi*hi, hi*mid, mid*hi, hi*lo, mid*mid, lo*hi; the dropped ones are
+// ~2^-24 relative).  BF16 has FP32's exponent range; the same per-transform
+// scaling is applied anyway, so both variants share the data path.
+//
+// Operand layouts (v_mfma_f32_16x16x32_f16 / _bf16, wave64): lane l holds row l%16 of
 // A and column l%16 of B, eight consecutive K values starting at 8*(l/16);
 // the result holds column l%16, rows 4*(l/16)..+3.  Only the result layout
 // matters for correctness (A and B share one K order); it is checked on the
@@ -76,6 +83,35 @@ __global__ void gen_mfma_mats_kernel(h8* out) {
   out[mat_index(mat, mtile, 1, lane)] = lo;
 }
 
+// BF16 table: [matrix][mtile][hi/mid/lo][lane] -> b8
+__host__ __device__ constexpr int mat_index3(int mat, int mtile, int part, int lane) {
+  return ((mat * 2 + mtile) * 3 + part) * 64 + lane;
+}
+
+__global__ void gen_mfma_mats_bf16_kernel(b8* out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // (mat, mtile, lane)
+  if (idx >= kMats * 2 * 64) return;
+  const int lane = idx & 63;
+  const int mtile = (idx >> 6) & 1;
+  const int mat = idx >> 7;
+  const int m = lane & 15;
+  const int k0 = 8 * (lane >> 4);
+  b8 hi, mid, lo;
+  for (int i = 0; i < 8; i++) {
+    const double a = a_value(mat, mtile, m, k0 + i);
+    const __bf16 h = (__bf16)(float)a;
+    const double r1 = a - (double)(float)h;
+    const __bf16 md = (__bf16)(float)r1;
+    const double r2 = r1 - (double)(float)md;
+    hi[i] = h;
+    mid[i] = md;
+    lo[i] = (__bf16)(float)r2;
+  }
+  out[mat_index3(mat, mtile, 0, lane)] = hi;
+  out[mat_index3(mat, mtile, 1, lane)] = mid;
+  out[mat_index3(mat, mtile, 2, lane)] = lo;
+}
+
 
 
 // FP16 hi/lo split of 8 values (already scaled into FP16 range): hi = x
@@ -106,6 +142,55 @@ __device__ __forceinline__ AOps load_aops(const h8* __restrict__ mats, int mat, 
           mats[mat_index(mat, 1, 1, lane)]};
 }
 
+// BF16: three-way split of 8 values, x = hi + mid + lo (each remainder exact
+// in FP32, each part rounded to BF16).
+__device__ __forceinline__ void split8_bf16(const float (&x)[8], b8& hi, b8& mid, b8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const __bf16 h = (__bf16)x[i];
+    const float r1 = x[i] - (float)h;
+    const __bf16 m = (__bf16)r1;
+    hi[i] = h;
+    mid[i] = m;
+    lo[i] = (__bf16)(r1 - (float)m);
+  }
+}
+
+struct AOps3 {
+  b8 h[2], m[2], l[2];  // per M-tile (Re rows, Im rows): hi, mid, lo
+};
+
+__device__ __forceinline__ AOps3 load_aops3(const b8* __restrict__ mats, int mat, int lane) {
+  AOps3 A;
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    A.h[t] = mats[mat_index3(mat, t, 0, lane)];
+    A.m[t] = mats[mat_index3(mat, t, 1, lane)];
+    A.l[t] = mats[mat_index3(mat, t, 2, lane)];
+  }
+  return A;
+}
+
+__device__ __forceinline__ void mfma_group3(const AOps3& A, const float (&x)[8], f4& re, f4& im) {
+  b8 bh, bm, bl;
+  split8_bf16(x, bh, bm, bl);
+  f4 acc[2];
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    // the 2^-16 terms, the two 2^-8 terms, then hi*hi
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.h[t], bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.m[t], bm, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.l[t], bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.h[t], bm, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.m[t], bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.h[t], bh, c, 0, 0, 0);
+    acc[t] = c;
+  }
+  re = acc[0];
+  im = acc[1];
+}
+
 // One group: 16 butterflies (B columns), operands already in registers and
 // already scaled into FP16 range.
 __device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f4& re, f4& im) {
@@ -124,6 +209,7 @@ __device__ __forceinline__ void mfma_group(const AOps& A, const float (&x)[8], f
 // 16 transforms of 256 points per work-group of 256 threads (4 waves).  The
 // tile lives in LDS as separate re / im planes so each lane reads exactly the
 // components its B-operand quarter needs.
+template <bool BF>
 __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   __shared__ float lre[kMCT * kMLP];
   __shared__ float lim[kMCT * kMLP];
@@ -140,8 +226,11 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   const idx_t ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2;
 
   // stage-1 A operands issued before the tile load so their latency overlaps
-  // it; stage 2's are issued once the tile is in LDS (overlapping stage 1)
-  const AOps A1 = load_aops(p.mats, 0, lane);
+  // it; stage 2's are issued once the tile is in LDS (overlapping stage 1);
+  // the BF16 operands (three parts) are loaded per group instead
+  const b8* mats3 = reinterpret_cast<const b8*>(p.mats);
+  AOps A1{};
+  if constexpr (!BF) A1 = load_aops(p.mats, 0, lane);
 
   // load: 16 x 256 complex, one transform per thread and 16 of its points;
   // columns contiguous in memory: 16 lanes read one 128-byte row segment;
@@ -181,8 +270,10 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
   }
   __syncthreads();
   AOps A2[4];
+  if constexpr (!BF) {
 #pragma unroll
-  for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
+    for (int g = 0; g < 4; g++) A2[g] = load_aops(p.mats, 1 + wave * 4 + g, lane);
+  }
   auto shift_of = [&](int c) -> int {
     return transform_shift(fmaxf(fmaxf(smax[0][c], smax[1][c]), fmaxf(smax[2][c], smax[3][c])));
   };
@@ -203,7 +294,10 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
 #pragma unroll
     for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)] * sc;
     f4 re, im;
-    mfma_group(A1, x, re, im);
+    if constexpr (BF)
+      mfma_group3(load_aops3(mats3, 0, lane), x, re, im);
+    else
+      mfma_group(A1, x, re, im);
     // the MFMA consumed every lane's loads, so the column can be overwritten
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
@@ -224,7 +318,10 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
 #pragma unroll
     for (int i = 0; i < 8; i++) x[i] = plane[c * kMLP + j + 16 * (rb + i)];
     f4 re, im;
-    mfma_group(A2[g], x, re, im);
+    if constexpr (BF)
+      mfma_group3(load_aops3(mats3, 1 + wave * 4 + g, lane), x, re, im);
+    else
+      mfma_group(A2[g], x, re, im);
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) {
       const int m = 4 * q + jj;
@@ -260,8 +357,8 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
 }
 
 // Layout self-test: D = A * B for A 16 x 32 (rows 16), B 32 x 16 with small
-// integers packed the way mfma_group packs them; D read back with the assumed
-// result layout.  out[0] receives the number of mismatching entries.
+// integers packed the way mfma_group packs them, through both the FP16 and the
+// BF16 instruction; D read back with the assumed result layout.  out[0] receives the number of mismatching entries.
 __global__ void mfma_selftest_kernel(int* out) {
   const int lane = threadIdx.x;
   h8 a, b;
@@ -271,8 +368,14 @@ __global__ void mfma_selftest_kernel(int* out) {
     a[i] = (_Float16)(float)(((m * 7 + k * 3) % 11) - 5);
     b[i] = (_Float16)(float)(((k * 5 + m * 2) % 13) - 6);  // here m is the B column
   }
+  b8 ab, bb;
+  for (int i = 0; i < 8; i++) {
+    ab[i] = (__bf16)(float)a[i];
+    bb[i] = (__bf16)(float)b[i];
+  }
   f4 z = {0.f, 0.f, 0.f, 0.f};
-  f4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, z, 0, 0, 0);
+  const f4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, z, 0, 0, 0);
+  const f4 db = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bb, z, 0, 0, 0);
   int bad = 0;
   for (int jj = 0; jj < 4; jj++) {
     const int row = 4 * (lane >> 4) + jj;
@@ -281,19 +384,26 @@ __global__ void mfma_selftest_kernel(int* out) {
     for (int k = 0; k < 32; k++)
       ref += (float)(((row * 7 + k * 3) % 11) - 5) * (float)(((k * 5 + col * 2) % 13) - 6);
     if (d[jj] != ref) bad++;
+    if (db[jj] != ref) bad++;
   }
   atomicAdd(out, bad);
 }
 
-hipError_t gen_mfma_mats(void* out, hipStream_t st) {
-  hipLaunchKernelGGL(gen_mfma_mats_kernel, dim3((kMats * 2 * 64 + 255) / 256), dim3(256), 0, st, (h8*)out);
+hipError_t gen_mfma_mats(void* out, int bf16, hipStream_t st) {
+  if (bf16)
+    hipLaunchKernelGGL(gen_mfma_mats_bf16_kernel, dim3((kMats * 2 * 64 + 255) / 256), dim3(256), 0, st, (b8*)out);
+  else
+    hipLaunchKernelGGL(gen_mfma_mats_kernel, dim3((kMats * 2 * 64 + 255) / 256), dim3(256), 0, st, (h8*)out);
   return hipGetLastError();
 }
-size_t mfma_mats_bytes() { return sizeof(h8) * (size_t)kMats * 2 * 2 * 64; }
+size_t mfma_mats_bytes(int bf16) { return sizeof(h8) * (size_t)kMats * 2 * (bf16 ? 3 : 2) * 64; }
 
 hipError_t launch_mfma_dft256(const MfmaParams& p, int n_tiles, hipStream_t st) {
   if (n_tiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(mfma_dft256_kernel, dim3(n_tiles), dim3(256), 0, st, p);
+  if (p.bf16)
+    hipLaunchKernelGGL(mfma_dft256_kernel<true>, dim3(n_tiles), dim3(256), 0, st, p);
+  else
+    hipLaunchKernelGGL(mfma_dft256_kernel<false>, dim3(n_tiles), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 

@@ -204,9 +204,9 @@ const void* get_table(int type, idx_t n, int bits) {
     size_t bytes = sizeof(cx<T>) * (size_t)(off > 0 ? off : 1);
     e = hipMalloc(&p, bytes);
     if (e == hipSuccess) e = gen_stage_tw<T>((cx<T>*)p, sl, nullptr);
-  } else if (type == 3) {  // A operands of the MFMA DFT-256 tile
-    e = hipMalloc(&p, mfma_mats_bytes());
-    if (e == hipSuccess) e = gen_mfma_mats(p, nullptr);
+  } else if (type == 3) {  // A operands of the MFMA DFT-256 tile (bits: 1 = BF16 three-way)
+    e = hipMalloc(&p, mfma_mats_bytes(bits));
+    if (e == hipSuccess) e = gen_mfma_mats(p, bits, nullptr);
   } else if (type == 1) {  // four-step twiddles, hi then lo
     idx_t nhi = n >> bits, nlo = idx_t(1) << bits;
     e = hipMalloc(&p, sizeof(cx<T>) * (size_t)(nhi + nlo));
@@ -383,10 +383,11 @@ struct Builder {
     // four-step along the axis: n = n1 * n2
     idx_t n2, n1;
     if (inner == 1) {
-      // 1024-point rows up to 2^21; beyond, 2048-point rows keep the column
-      // pass at <= 2048 points, whose tiles still span 16 columns
-      n2 = n <= ((idx_t)1 << 21) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
-      if (P.mode == afft_gpu_mode_mfma_f16 && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
+      // 1024-point rows up to 2^20; beyond, 2048-point rows keep the column
+      // pass at <= 2048 points (2^21 as 1024 x 2048 measured 3-5 % faster
+      // than 2048 x 1024)
+      n2 = n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
+      if (P.mode != afft_gpu_mode_fma && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
       if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
         set_err("transform length %lld is too long", (long long)n);
         ok = false;
@@ -654,10 +655,10 @@ struct Builder {
       q.out_nt = P.var.stream_io && q.out.role == R_DST;
       q.xcd = P.var.xcd_order;
       q.mfma = 0;
-      if (P.mode == afft_gpu_mode_mfma_f16 && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
+      if (P.mode != afft_gpu_mode_fma && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
           (q.epi == EPI_PLAIN || q.epi == EPI_TW4) && q.in.il && q.out.il) {
-        q.mfma = 1;
-        q.mfma_mats = get_table<T>(3, 256, 0);
+        q.mfma = P.mode;
+        q.mfma_mats = get_table<T>(3, 256, P.mode == afft_gpu_mode_mfma_bf16 ? 1 : 0);
         if (!q.mfma_mats) ok = false;
       }
     }
@@ -987,6 +988,7 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     m.in_nt = r.in_nt;
     m.out_nt = r.out_nt;
     m.mats = (const h8*)r.mfma_mats;
+    m.bf16 = r.mfma == afft_gpu_mode_mfma_bf16;
     if (r.epi == EPI_TW4) {
       m.tw4_hi = (const cx<float>*)r.tw4_hi;
       m.tw4_lo = (const cx<float>*)r.tw4_lo;
@@ -1571,7 +1573,7 @@ int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode) {
   Header h;
   if (!read_header(plan, h)) return -1;
   Plan& P = *h.plan;
-  if (mode != afft_gpu_mode_fma && mode != afft_gpu_mode_mfma_f16) {
+  if (mode != afft_gpu_mode_fma && mode != afft_gpu_mode_mfma_f16 && mode != afft_gpu_mode_mfma_bf16) {
     set_err("unknown mode %d", mode);
     return -1;
   }
@@ -1582,9 +1584,9 @@ int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode) {
     rebuild_any(P);
     return -1;
   }
-  if (mode == afft_gpu_mode_mfma_f16) {
+  if (mode != afft_gpu_mode_fma) {
     int used = 0;
-    for (int i = 0; i < P.npass; i++) used += P.pass[i].mfma;
+    for (int i = 0; i < P.npass; i++) used += P.pass[i].mfma != 0;
     if (!used) {
       set_err("no pass of this plan can run on the MFMA tile (needs FP32 c2c, interleaved, 256-point passes)");
       P.mode = old;
@@ -1645,7 +1647,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   if (r.type == PT_FFT)
     snprintf(buf, len,
              "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.mfma ? "[mfma]" : (P.pipe && i < 2 ? "[fused]" : ""), r.L, r.CT, r.sv ? " sv1" : "",
+             r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]") : (P.pipe && i < 2 ? "[fused]" : ""),
+             r.L, r.CT, r.sv ? " sv1" : "",
              r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
              r.store_t ? " store_t" : "", (long long)r.n_col,
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);

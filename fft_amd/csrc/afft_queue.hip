@@ -369,7 +369,7 @@ struct Queue {
   int grid = 0;
   int ntasks = 0;
   int nchain = 0;                   // jobs run as tile chains
-  int groups_per_cu = 2;            // persistent work-groups per CU (2 or 4)
+  int groups_per_cu = 4;            // persistent work-groups per CU (2 or 4; 4 measured faster)
   int ncu = 256;
   long long spin_limit = 1LL << 26;  // dependency-wait iterations before the queue is flagged
   QTask* d_tasks = nullptr;

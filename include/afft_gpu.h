@@ -29,8 +29,9 @@ enum { afft_gpu_c = 0, afft_gpu_ci = 1, afft_gpu_r = 2, afft_gpu_ri = 3 };
 /* arithmetic modes of FP32 complex plans */
 enum
 {
-  afft_gpu_mode_fma = 0,      /* FP32 FMA Stockham passes (default) */
-  afft_gpu_mode_mfma_f16 = 1  /* 256-point passes as FP16 hi+lo MFMA products, FP32 accumulation */
+  afft_gpu_mode_fma = 0,       /* FP32 FMA Stockham passes (default) */
+  afft_gpu_mode_mfma_f16 = 1,  /* 256-point passes as FP16 hi+lo MFMA products, FP32 accumulation */
+  afft_gpu_mode_mfma_bf16 = 2  /* the same with BF16 hi+mid+lo operands (six products) */
 };
 
 typedef struct afft_gpu_plan_struct afft_gpu_plan;
@@ -167,8 +168,9 @@ int afft_gpu_queue_status(const afft_gpu_queue* q);
 /* Iterations a tile waits for the tiles it depends on before it flags the
  * queue and proceeds (default 2^26, about a second). */
 int afft_gpu_queue_set_spin_limit(afft_gpu_queue* q, long long limit);
-/* Persistent work-groups per CU: 2 (default; registers for the largest
- * tiles) or 4 (twice the tiles in flight, registers bounded to 128). */
+/* Persistent work-groups per CU: 4 (default; registers bounded to 128 per
+ * lane, some spills) or 2 (registers for the largest tiles, half the tiles in
+ * flight). */
 int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* q, int groups);
 void afft_gpu_queue_destroy(afft_gpu_queue* q);
 

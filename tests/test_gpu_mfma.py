@@ -1,6 +1,7 @@
-"""GPU tests of the split-precision MFMA mode (FP16 hi+lo operands, FP32
-accumulation) against the FP64 oracle / device FP64 plans, at the same bound
-as the FP32 path: rel-L2 <= 2*log2(N)*2^-24 per transform."""
+"""GPU tests of the split-precision MFMA modes (FP16 hi+lo and BF16
+hi+mid+lo operands, FP32 accumulation) against the FP64 oracle / device FP64
+plans, at the same bound as the FP32 path: rel-L2 <= 2*log2(N)*2^-24 per
+transform."""
 import sys
 
 import numpy as np
@@ -29,11 +30,15 @@ def test_mfma_operand_layout(G):
     assert _lib.lib().afft_gpu_mfma_selftest() == 0
 
 
+MODES = {1: "[mfma]", 2: "[mfma-bf16]"}
+
+
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("n,batch", [(256, 64), (65536, 8)])
-def test_mfma_c2c_scaled_amplitudes(G, n, batch):
+def test_mfma_c2c_scaled_amplitudes(G, n, batch, mode):
     import torch
     from fft_amd.afft import Plan
-    say("mfma c2c", n, batch)
+    say("mfma c2c", n, batch, "mode", mode)
     x = O.complex_input(11, (batch, n)).astype(np.complex128)
     amp = 10.0 ** (6 * O.uniform(12, batch).astype(np.float64) / 2)  # 10^u, u in [-3, 3)
     x = x * amp[:, None]
@@ -42,8 +47,8 @@ def test_mfma_c2c_scaled_amplitudes(G, n, batch):
     for inverse in (False, True):
         dst = G.empty_dev(2 * n * batch)
         p = Plan("ci" if inverse else "c", (n,), batch)
-        p.set_mode(1)
-        assert any("[mfma]" in d for d in p.describe())
+        p.set_mode(mode)
+        assert any(MODES[mode] in d for d in p.describe())
         p(src, None, dst, None)
         torch.cuda.synchronize()
         y = G.to_host(dst)
@@ -55,10 +60,11 @@ def test_mfma_c2c_scaled_amplitudes(G, n, batch):
         p.close()
 
 
-def test_mfma_3d_256_cube(G):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mfma_3d_256_cube(G, mode):
     import torch
     from fft_amd.afft import Plan
-    say("mfma 256^3")
+    say("mfma 256^3 mode", mode)
     shape = (256, 256, 256)
     n = 256 ** 3
     g = torch.Generator(device="cuda").manual_seed(3)
@@ -66,8 +72,8 @@ def test_mfma_3d_256_cube(G):
     y = torch.empty_like(x)
     y64 = torch.empty(2 * n, device="cuda", dtype=torch.float64)
     p = Plan("c", shape)
-    p.set_mode(1)
-    assert sum("[mfma]" in d for d in p.describe()) == 3
+    p.set_mode(mode)
+    assert sum(MODES[mode] in d for d in p.describe()) == 3
     p(x, None, y, None)
     Plan("c", shape, dtype=np.float64)(x.double(), None, y64, None)
     torch.cuda.synchronize()
@@ -79,3 +85,7 @@ def test_mfma_mode_rejected_where_it_cannot_apply(G):
     p = Plan("c", (1024,), 4)
     with pytest.raises(RuntimeError):
         p.set_mode(1)
+    with pytest.raises(RuntimeError):
+        p.set_mode(2)
+    with pytest.raises(RuntimeError):
+        p.set_mode(3)
