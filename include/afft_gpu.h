@@ -76,11 +76,12 @@ int afft_gpu_set_stream(void* state, void* stream);
  * (0 = automatic).  Tuning knob; the result does not depend on it. */
 int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk);
 
-/* Selects the arithmetic mode.  afft_gpu_mode_mfma_f16 runs every 256-point
- * pass of an FP32 interleaved c2c plan (1-D N = 256 or 65536 = 256 x 256, and
- * 256-long axes of N-D shapes) on the matrix cores: each radix-16 stage is a
- * product of FP16 hi+lo operands (per-butterfly power-of-two scaling) with
- * FP32 accumulation.  Returns -1 if no pass of the plan qualifies. */
+/* Selects the arithmetic mode.  afft_gpu_mode_mfma_f16 / _bf16 run every
+ * 256-point pass of an FP32 interleaved c2c plan (1-D N = 256 or 65536 =
+ * 256 x 256, and 256-long axes of N-D shapes) on the matrix cores: each
+ * radix-16 stage is a product of split operands (FP16 hi+lo, or BF16
+ * hi+mid+lo) with FP32 accumulation, after a per-transform power-of-two
+ * scaling.  Returns -1 if no pass of the plan qualifies or the mode is unknown. */
 int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode);
 
 /* Runs a device check of the MFMA operand layout the mode relies on; returns
