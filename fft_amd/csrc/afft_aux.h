@@ -28,6 +28,22 @@ struct RealRowsArgs {
   const cx<T>* real_tw;  // W_{2h}^k, k < h
 };
 
+// Forward real pass that also undoes the four-step transposition: the input
+// is Y[b][k1][k2] (n1 x n2, k2 contiguous, interleaved) holding the h-point
+// c2c result Z[k1 + n1*k2]; the output is the h+1 bins of the r2c in natural
+// order.
+template <typename T>
+struct RealTrArgs {
+  const cx<T>* in;
+  T* out_re;
+  T* out_im;
+  int out_il;
+  idx_t in_bs, out_bs;  // batch strides (complex / scalar elements)
+  int n1, n2;
+  idx_t h, nbatch;
+  const cx<T>* real_tw;  // W_{2h}^k, k < h
+};
+
 template <typename T>
 hipError_t gen_stage_tw(cx<T>* out, const StageList& sl, hipStream_t st);
 template <typename T>
@@ -36,6 +52,8 @@ template <typename T>
 hipError_t gen_real_tw(cx<T>* out, idx_t L, hipStream_t st);
 template <typename T>
 hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st);
+template <typename T>
+hipError_t real_tr(const RealTrArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

@@ -159,6 +159,81 @@ hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Transposing forward real pass.  A work-group takes a block of 32 x 32
+// (k1, k2) entries: it reads Y[k1][k2] and the partner Z[h - k] (row n1 - k1,
+// column n2 - 1 - k2; row 0: column (n2 - k2) mod n2), both as 256-byte row
+// segments, forms X[k] = (Z[k] + conj Z[h-k])/2 - i (Z[k] - conj Z[h-k]) W_2h^k
+// / 2 (onedim.hpp:1066-1106 arithmetic, W_2h^k = W_2h^k1 * W_2h^(n1 k2)), and
+// writes the bins k1 + n1*k2 through LDS as 256-byte runs of consecutive k1.
+// Bin 0 and bin h come from Z[0] (its block writes both).
+template <typename T>
+__global__ void __launch_bounds__(256) real_tr_kernel(RealTrArgs<T> a) {
+  constexpr int TB = 32;
+  __shared__ cx<T> sx[TB][TB + 1];
+  const int nb1 = a.n1 / TB, nb2 = a.n2 / TB;
+  int bid = blockIdx.x;
+  const idx_t b = bid / (nb1 * nb2);
+  bid -= (int)(b * (nb1 * nb2));
+  const int a0 = (bid % nb1) * TB, c0 = (bid / nb1) * TB;
+  const cx<T>* Y = a.in + b * a.in_bs;
+  const idx_t h = a.h;
+#pragma unroll
+  for (int i = 0; i < TB * TB / 256; i++) {
+    const int e = threadIdx.x + 256 * i;
+    const int r = e / TB, q = e % TB;
+    const int k1 = a0 + r, k2 = c0 + q;
+    const cx<T> z = Y[(idx_t)k1 * a.n2 + k2];
+    const int p1 = k1 == 0 ? 0 : a.n1 - k1;
+    const int p2 = k1 == 0 ? ((a.n2 - k2) & (a.n2 - 1)) : a.n2 - 1 - k2;
+    const cx<T> zp = Y[(idx_t)p1 * a.n2 + p2];
+    cx<T> x;
+    if (k1 == 0 && k2 == 0) {
+      x = cx<T>{z.x + z.y, T(0)};
+      const cx<T> nyq{z.x - z.y, T(0)};
+      const idx_t o = b * a.out_bs + h;
+      if (a.out_il) {
+        reinterpret_cast<cx<T>*>(a.out_re)[o] = nyq;
+      } else {
+        a.out_re[o] = nyq.x;
+        a.out_im[o] = nyq.y;
+      }
+    } else {
+      const cx<T> w = cmul(a.real_tw[k1], a.real_tw[(idx_t)a.n1 * k2]);
+      const cx<T> s1c = conjg(zp);
+      const T hf = T(0.5);
+      const cx<T> aa = {(z.x + s1c.x) * hf, (z.y + s1c.y) * hf};
+      const cx<T> bw = cmul(z - s1c, w);
+      const cx<T> bb = {bw.x * hf, bw.y * hf};
+      x = aa + mul_neg_i(bb);
+    }
+    sx[q][r] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < TB * TB / 256; i++) {
+    const int e = threadIdx.x + 256 * i;
+    const int r = e % TB, q = e / TB;
+    const idx_t o = b * a.out_bs + (idx_t)(a0 + r) + (idx_t)a.n1 * (c0 + q);
+    const cx<T> x = sx[q][r];
+    if (a.out_il) {
+      reinterpret_cast<cx<T>*>(a.out_re)[o] = x;
+    } else {
+      a.out_re[o] = x.x;
+      a.out_im[o] = x.y;
+    }
+  }
+}
+
+template <typename T>
+hipError_t real_tr(const RealTrArgs<T>& a, hipStream_t st) {
+  if (a.n1 % 32 || a.n2 % 32 || a.n1 < 32 || a.n2 < 32) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (idx_t)(a.n1 / 32) * (a.n2 / 32);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(real_tr_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 // Single-value real transform (total size 1): X[0] = (x, 0) / x = Re X[0].
 template <typename T>
 __global__ void real_one_kernel(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s,
@@ -252,6 +327,8 @@ template hipError_t gen_real_tw<float>(cx<float>*, idx_t, hipStream_t);
 template hipError_t gen_real_tw<double>(cx<double>*, idx_t, hipStream_t);
 template hipError_t real_rows<float>(const RealRowsArgs<float>&, bool, hipStream_t);
 template hipError_t real_rows<double>(const RealRowsArgs<double>&, bool, hipStream_t);
+template hipError_t real_tr<float>(const RealTrArgs<float>&, hipStream_t);
+template hipError_t real_tr<double>(const RealTrArgs<double>&, hipStream_t);
 template hipError_t real_one<float>(const float*, float*, float*, int, int, idx_t, idx_t, idx_t, hipStream_t);
 template hipError_t real_one<double>(const double*, double*, double*, int, int, idx_t, idx_t, idx_t, hipStream_t);
 template hipError_t err_sums<float>(const float*, const double*, idx_t, double*, hipStream_t);

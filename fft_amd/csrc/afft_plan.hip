@@ -69,7 +69,7 @@ constexpr int kMaxDim = 64;
 constexpr int kMaxPass = 160;
 
 enum Role { R_SRC = 0, R_DST = 1, R_SCR = 2, R_SCR2 = 3 };
-enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2 };
+enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3 };
 
 struct Side {
   int role = R_SRC;
@@ -495,6 +495,68 @@ struct Builder {
     }
   }
 
+  // 1-D r2c with h = n/2 > the longest row pass: four-step h = n1 * n2 whose
+  // pass B stores rows (k1-major, contiguous) into a second scratch, then the
+  // transposing real pass reads Z[k1 + n1*k2] = Y[k1][k2] and writes the h+1
+  // bins in natural order (one kernel for the transposition and the real
+  // post-pass instead of a transposed store and a separate real pass).
+  void r2c_fourstep_tr(const Side& in, const Side& out, idx_t h) {
+    const idx_t n2 = h <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, h / max_col_len<T>());
+    const idx_t n1 = h / n2;
+    if (n1 > max_col_len<T>() || n1 < 32) {
+      set_err("transform length %lld is too long", (long long)(2 * h));
+      ok = false;
+      return;
+    }
+    need_scratch(R_SCR, h);
+    need_scratch(R_SCR2, h);
+    const int bits = (ilog2(h) + 1) / 2;
+    const void* tw4 = get_table<T>(1, h, bits);
+    if (!tw4) {
+      ok = false;
+      return;
+    }
+    PassRec* a = add();
+    if (!a || !fft_pass(a, (int)n1, 1, EPI_TW4, 0)) return;
+    a->in = in;
+    a->in.s_j = n2;
+    a->in.s_c = 1;
+    a->in.s_b = in.s_u;
+    scratch_side(a->out, R_SCR);
+    a->out.s_j = n2;
+    a->out.s_c = 1;
+    a->out.s_b = h;
+    a->n_col = n2;
+    a->b_per_user = 1;
+    a->tw4_hi = tw4;
+    a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
+    a->tw4_bits = bits;
+    PassRec* b = add();
+    if (!b || !fft_pass(b, (int)n2, 0, EPI_PLAIN, 0)) return;
+    scratch_side(b->in, R_SCR);
+    b->in.s_j = 1;
+    b->in.s_c = n2;
+    b->in.s_b = h;
+    scratch_side(b->out, R_SCR2);
+    b->out.s_j = 1;
+    b->out.s_c = n2;
+    b->out.s_b = h;
+    b->n_col = n1;
+    b->b_per_user = 1;
+    PassRec* r = add();
+    if (!r) return;
+    r->type = PT_REALTR;
+    scratch_side(r->in, R_SCR2);
+    r->in.s_b = h;
+    r->out = out;
+    r->out.s_j = 1;
+    r->out.s_b = out.s_u;
+    r->rr_h = h;
+    r->rr_m = n1;
+    r->real_tw = get_table<T>(2, h, 0);
+    if (!r->real_tw) ok = false;
+  }
+
   // The inner c2c axes of an N-D r2c, on each of the h+1 rows of dst.
   void inner_axes_on_dst(const Side& usr_dst, const idx_t* d, int nd, idx_t h) {
     idx_t inner = 1;
@@ -737,11 +799,7 @@ struct Builder {
           r->fold_user = 1;
           return ok;
         }
-        axis(in, out, 1, h, 1, 0, 0);
-        Side rin = out, rout = out;
-        rin.s_j = rout.s_j = 1;
-        rin.s_b = rout.s_b = h + 1;
-        real_rows(rin, rout, h, 1, 0);
+        r2c_fourstep_tr(in, out, h);
         return ok;
       }
       if (h > kColDirectMax) {
@@ -1025,6 +1083,22 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.nbatch = cnt;
     a.real_tw = (const cx<T>*)r.real_tw;
     HIP_OK(real_rows<T>(a, r.rr_inv != 0, st));
+    return true;
+  }
+  if (r.type == PT_REALTR) {
+    RealTrArgs<T> a{};
+    a.in = reinterpret_cast<const cx<T>*>(ire);
+    a.out_re = ore;
+    a.out_im = oim;
+    a.out_il = r.out.il;
+    a.in_bs = r.in.s_b;
+    a.out_bs = r.out.s_b;
+    a.n1 = (int)r.rr_m;
+    a.n2 = (int)(r.rr_h / r.rr_m);
+    a.h = r.rr_h;
+    a.nbatch = cnt;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(real_tr<T>(a, st));
     return true;
   }
   // PT_REALONE
@@ -1652,6 +1726,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
              r.store_t ? " store_t" : "", (long long)r.n_col,
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
+  else if (r.type == PT_REALTR)
+    snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
   else

@@ -189,7 +189,8 @@ typedef struct
 
 typedef struct
 {
-  int32_t type; /* 0 = FFT pass, 1 = real-rows pass, 2 = single-value real */
+  int32_t type; /* 0 = FFT pass, 1 = real-rows pass, 2 = single-value real,
+                   3 = transposing real pass (rr_h = h, rr_m = n1) */
   int32_t L, CT, colmap, epi, store_t;
   afft_gpu_side_info in, out;
   int64_t n_col;

@@ -237,6 +237,29 @@ def _real_rows(p, bufs, start, cnt, pi):
         _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
 
 
+def _real_tr(p, bufs, start, cnt, pi):
+    """Transposing forward real pass (afft_aux.hip real_tr_kernel): input
+    Y[b][k1][k2] (n1 x n2) holds Z[k1 + n1*k2]; output bins 0..h natural."""
+    h, n1 = p.rr_h, p.rr_m
+    n2 = h // n1
+    b = np.arange(cnt).reshape(cnt, 1)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    k = np.arange(h).reshape(1, h)
+    k1, k2 = k % n1, k // n1
+    e_in = uin + b * p.in_.s_b + k1 * n2 + k2
+    Z = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    kk = np.arange(h)
+    Zh = Z[..., (-kk) % h]
+    w = np.exp(-2j * np.pi * kk / (2 * h))
+    X = 0.5 * (Z + np.conj(Zh)) - 1j * 0.5 * (Z - np.conj(Zh)) * w
+    X[..., 0] = Z[..., 0].real + Z[..., 0].imag
+    last = (Z[..., 0].real - Z[..., 0].imag)[..., None]
+    X = np.concatenate([X, last], axis=-1)
+    e_out = uout + b * p.out.s_b + np.arange(h + 1).reshape(1, h + 1) * p.out.s_j
+    _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
+
+
 def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32):
     """Executes the plan model; returns the dst arrays (float64)."""
     info, passes = inspect(kind, dims, fmt, batch, prec)
@@ -253,6 +276,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32):
                 _fft_pass(p, bufs, start, cnt, pi)
             elif p.type == 1:
                 _real_rows(p, bufs, start, cnt, pi)
+            elif p.type == 3:
+                _real_tr(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:
