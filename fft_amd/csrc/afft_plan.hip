@@ -799,7 +799,18 @@ struct Builder {
           r->fold_user = 1;
           return ok;
         }
-        r2c_fourstep_tr(in, out, h);
+        const idx_t n2 = h <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, h / max_col_len<T>());
+        if (h / n2 >= 32) {
+          r2c_fourstep_tr(in, out, h);
+          return ok;
+        }
+        // short column pass (FP64 h = 16384): four-step with a transposed
+        // store, then the real pass in place
+        axis(in, out, 1, h, 1, 0, 0);
+        Side rin = out, rout = out;
+        rin.s_j = rout.s_j = 1;
+        rin.s_b = rout.s_b = h + 1;
+        real_rows(rin, rout, h, 1, 0);
         return ok;
       }
       if (h > kColDirectMax) {
