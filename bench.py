@@ -13,7 +13,8 @@ The JSON line also carries
   cpu_baseline  afft itself (oracle/_ref, built from the reference sources) on
                 this host's cores over a bounded sample of the same workload;
   extra         the rest of the afft configs measured the same way (r2c
-                16 x 2^22, c2c 256^3, r2c 8192^2, the device-copy ceiling),
+                16 x 2^22, c2c 256^3, r2c 8192^2, FP64 c2c 16 x 2^20, the
+                device-copy ceiling),
                 the MFMA split-precision mode against the FP32-FMA plan on
                 4096 x 2^16 with 10^u amplitudes (time and error vs FP64),
                 and the 10,000-job uneven batch through the persistent work
@@ -335,6 +336,17 @@ def main():
         extra["r2c_8192^2_GFLOPs_2.5NlogN"] = round(2.5 * n2 * 26 / (ms2 * 1e-3) / 1e9, 1)
         extra["r2c_8192^2_hbm_frac"] = round((4 * n2 + 8 * 4097 * 8192) / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
         del x2, y2, p2
+        # FP64 c2c 16 x 2^20 (the afft_64_* path)
+        nd, bd = 1 << 20, 16
+        xd = torch.rand(2 * nd * bd, device=dev, generator=g, dtype=torch.float64).mul_(2).sub_(1)
+        yd = torch.empty_like(xd)
+        pd = Plan("c", (nd,), bd, dtype=np.float64)
+        pd(xd, None, yd, None, stream)
+        msd = time_events(torch, stream, lambda: pd(xd, None, yd, None, stream), 10)
+        extra["c2c_fp64_16x2^20_ms"] = round(msd, 4)
+        extra["c2c_fp64_16x2^20_GFLOPs"] = round(flops_c2c(nd, bd) / (msd * 1e-3) / 1e9, 1)
+        extra["c2c_fp64_16x2^20_hbm_frac"] = round(32 * nd * bd / (msd * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+        del xd, yd, pd
         torch.cuda.empty_cache()
         extra["mfma_vs_fp32"] = bench_mfma(torch, dev, stream, g)
         extra["uneven_queue"] = bench_queue(torch, dev, stream)

@@ -143,6 +143,11 @@ def test_plan_model_fp64_and_chunking():
     assert len(passes) == 2 and 1 <= info.chunk < 64
     err, info, _ = model_check("c", (1 << 15,), IL, 3, prec=64)
     assert err < 1e-9
+    # FP64 real transforms around the longest FP64 row pass (8192 complex)
+    for kind in ("r", "ri"):
+        for lg in (14, 15, 16, 17):
+            err, info, _ = model_check(kind, (1 << lg,), IL, 2, prec=64)
+            assert err < 1e-9, (kind, lg, err)
 
 
 def test_plan_shapes_of_headline_configs():
