@@ -213,12 +213,25 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
   }
 }
 
-template <int L, int CT, int SV = 0>
+// Row length of a transposed-store, half-exchange tile: the store phase reads
+// entry c2 * LL + k with c2 = lane % CT and k running over lane / CT, so LL is
+// padded to LL = 64 / CT (mod 32), which spreads the 64 lanes evenly over the
+// 32 LDS banks (two lanes per bank, the minimum for a wave).
+template <int L, int CT>
+constexpr int transposed_row_len() {
+  constexpr int base = LdsLen<L>::value;
+  constexpr int want = (64 / CT) % 32;
+  return base + ((want - base % 32) % 32 + 32) % 32;
+}
+
+// RT = 1: the tile is row-mapped with a transposed store and the half
+// exchange (its LDS rows use transposed_row_len).
+template <int L, int CT, int SV = 0, int RT = 0>
 struct PassShape {
   static constexpr int E = Sched<L, SV>::E;
   static constexpr int TPT = L / E;
   static constexpr int NT = CT * TPT;
-  static constexpr int LL = LdsLen<L>::value;
+  static constexpr int LL = RT ? transposed_row_len<L, CT>() : LdsLen<L>::value;
   static constexpr size_t lds_elems = size_t(CT) * LL;
   // LDS bytes of a tile of element type T (HX: one real per entry)
   template <typename T, bool HX>
@@ -282,7 +295,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
   static_assert(!(HX && EPI == EPI_R2C), "the real-output epilogue exchanges whole complex entries");
-  using PS = PassShape<L, CT, SV>;
+  using PS = PassShape<L, CT, SV, (!COLMAP && STORE_T && HX) ? 1 : 0>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
   constexpr int LL = PS::LL;
@@ -512,15 +525,21 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
 // Minimum waves per SIMD the compiler must allow for: half-exchange tiles
 // that fit twice in a CU's 160 KiB of LDS are meant to run two groups per CU,
 // so their registers are bounded accordingly; everything else runs one.
-template <typename T, int L, int CT, int SV, bool HX>
+// LDS bytes of a pass tile (the shape fft_tile uses).
+template <typename T, int L, int CT, bool COLMAP, bool STORE_T, int SV, bool HX>
+constexpr size_t tile_lds_bytes() {
+  return PassShape<L, CT, SV, (!COLMAP && STORE_T && HX) ? 1 : 0>::template lds_bytes<T, HX>();
+}
+
+template <typename T, int L, int CT, bool COLMAP, bool STORE_T, int SV, bool HX>
 constexpr int pass_min_waves() {
   constexpr int nt = PassShape<L, CT, SV>::NT;
-  constexpr int groups = (HX && 2 * PassShape<L, CT, SV>::template lds_bytes<T, HX>() <= 160 * 1024) ? 2 : 1;
+  constexpr int groups = (HX && 2 * tile_lds_bytes<T, L, CT, COLMAP, STORE_T, SV, HX>() <= 160 * 1024) ? 2 : 1;
   return groups * nt / 256 > 1 ? groups * nt / 256 : 1;
 }
 
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST, int SV, bool HX>
-__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (pass_min_waves<T, L, CT, SV, HX>()))
+__global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (pass_min_waves<T, L, CT, COLMAP, STORE_T, SV, HX>()))
     fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
   const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
