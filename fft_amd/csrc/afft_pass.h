@@ -213,25 +213,27 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
   }
 }
 
-// Row length of a transposed-store, half-exchange tile: the store phase reads
-// entry c2 * LL + k with c2 = lane % CT and k running over lane / CT, so LL is
-// padded to LL = 64 / CT (mod 32), which spreads the 64 lanes evenly over the
-// 32 LDS banks (two lanes per bank, the minimum for a wave).
-template <int L, int CT>
+// Row length (in LDS entries of EB bytes) of a transposed-store tile: the
+// store phase reads entry c2 * LL + k with c2 = lane % CT and k running over
+// lane / CT, so LL is padded to LL = 64 / CT modulo the entries per bank sweep
+// (128 / EB), which spreads the 64 lanes evenly over the 32 four-byte LDS
+// banks (the minimum number of cycles for a wave).
+template <int L, int CT, int EB>
 constexpr int transposed_row_len() {
   constexpr int base = LdsLen<L>::value;
-  constexpr int want = (64 / CT) % 32;
-  return base + ((want - base % 32) % 32 + 32) % 32;
+  constexpr int mod = EB > 0 ? 128 / EB : 1;
+  constexpr int want = (64 / CT) % mod;
+  return base + ((want - base % mod) % mod + mod) % mod;
 }
 
-// RT = 1: the tile is row-mapped with a transposed store and the half
-// exchange (its LDS rows use transposed_row_len).
+// RT > 0: the tile is row-mapped with a transposed store and LDS entries of
+// RT bytes (its rows use transposed_row_len).
 template <int L, int CT, int SV = 0, int RT = 0>
 struct PassShape {
   static constexpr int E = Sched<L, SV>::E;
   static constexpr int TPT = L / E;
   static constexpr int NT = CT * TPT;
-  static constexpr int LL = RT ? transposed_row_len<L, CT>() : LdsLen<L>::value;
+  static constexpr int LL = RT ? transposed_row_len<L, CT, RT>() : LdsLen<L>::value;
   static constexpr size_t lds_elems = size_t(CT) * LL;
   // LDS bytes of a tile of element type T (HX: one real per entry)
   template <typename T, bool HX>
@@ -292,10 +294,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
 
 // One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
 // tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
+// A cache policy known only at run time (one branch per access).
+struct RtPol {
+  int nt;
+};
+
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
   static_assert(!(HX && EPI == EPI_R2C), "the real-output epilogue exchanges whole complex entries");
-  using PS = PassShape<L, CT, SV, (!COLMAP && STORE_T && HX) ? 1 : 0>;
+  using PS = PassShape<L, CT, SV, (!COLMAP && STORE_T) ? (HX ? (int)sizeof(T) : 2 * (int)sizeof(T)) : 0>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
   constexpr int LL = PS::LL;
@@ -354,55 +361,76 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       }
     }
   }
-  // element j of this column / bin k of this column
-  auto load = [&](int j) AFGPU_INLINE -> cx<T> {
+  // element j of this column / bin k of this column.  POL is the cache policy
+  // of the fast path's buffer accesses (2 = non-temporal), chosen once per
+  // phase below so that no access carries a branch of its own.
+  auto load = [&](auto polc, int j) AFGPU_INLINE -> cx<T> {
     if constexpr (FAST) {
       const unsigned off = in0 + (unsigned)j * js8;
-      return p.in_nt ? buf_ld<T, 2>(rin, off) : buf_ld<T, 0>(rin, off);
+      return buf_ld<T, decltype(polc)::value>(rin, off);
     } else {
       return ld_elem(in_re, in_im, p.in_il, ib + (idx_t)j * p.in_js, p.in_nt);
     }
   };
-  auto store = [&](int k, cx<T> z) AFGPU_INLINE {
+  auto store = [&](auto polc, int k, cx<T> z) AFGPU_INLINE {
     if (p.swap_out) z = swapri(z);
     if constexpr (FAST) {
       const unsigned off = out0 + (unsigned)k * ks8;
-      if (p.out_nt)
-        buf_st<2>(z, rout, off);
-      else
-        buf_st<0>(z, rout, off);
+      if constexpr (std::is_same_v<decltype(polc), RtPol>) {
+        if (polc.nt)
+          buf_st<2>(z, rout, off);
+        else
+          buf_st<0>(z, rout, off);
+      } else {
+        buf_st<decltype(polc)::value>(z, rout, off);
+      }
     } else {
       st_elem(out_re, out_im, p.out_il, ob + (idx_t)k * p.out_ks, z, p.out_nt);
     }
   };
 
+  // runs f(pol) with pol = the compile-time cache policy for a side whose
+  // streaming flag is nt
+  auto with_pol = [&](bool nt, auto&& f) AFGPU_INLINE {
+    if constexpr (FAST) {
+      if (nt)
+        f(std::integral_constant<int, 2>{});
+      else
+        f(std::integral_constant<int, 0>{});
+    } else {
+      f(std::integral_constant<int, 0>{});
+    }
+  };
+
   cx<T> v[E];
   if (active) {
-    if constexpr (EPI == EPI_C2R) {
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        const int k = t + m * TPT;
-        cx<T> s0 = load(k);
-        cx<T> s1 = load(L - k);
-        cx<T> z;
-        if (k == 0) {
-          z = {s0.x + s1.x, s0.x - s1.x};
-        } else {
-          cx<T> w = p.real_tw[k];
-          cx<T> s1c = conjg(s1);
-          cx<T> a = s0 + s1c;
-          cx<T> bb = cmulc(s1c - s0, w);
-          z = a + mul_neg_i(bb);
-        }
-        v[m] = p.swap_in ? swapri(z) : z;
-      });
-    } else {
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        cx<T> z = load(t + m * TPT);
-        v[m] = p.swap_in ? swapri(z) : z;
-      });
-    }
+    with_pol(p.in_nt, [&](auto pol) AFGPU_INLINE {
+      if constexpr (EPI == EPI_C2R) {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          const int k = t + m * TPT;
+          cx<T> s0 = load(pol, k);
+          cx<T> s1 = load(pol, L - k);
+          cx<T> z;
+          if (k == 0) {
+            z = {s0.x + s1.x, s0.x - s1.x};
+          } else {
+            cx<T> w = p.real_tw[k];
+            cx<T> s1c = conjg(s1);
+            cx<T> a = s0 + s1c;
+            cx<T> bb = cmulc(s1c - s0, w);
+            z = a + mul_neg_i(bb);
+          }
+          v[m] = p.swap_in ? swapri(z) : z;
+        });
+      } else {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          cx<T> z = load(pol, t + m * TPT);
+          v[m] = p.swap_in ? swapri(z) : z;
+        });
+      }
+    });
   } else {
     static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = cx<T>{T(0), T(0)}; });
   }
@@ -412,124 +440,139 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   else
     run_stages<T, L, SV, 0, false>(v, t, lat, p.stage_tw);
 
-  if constexpr (EPI == EPI_R2C) {
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      lat(t + m * TPT) = v[m];
-    });
-    __syncthreads();
-    if (active) {
+  // The output policy is hoisted over the whole epilogue for column tiles
+  // (fewer live registers in their twiddled store); row tiles keep one branch
+  // per store, which measured fewer spills in the transposed store.
+  auto with_out_pol = [&](auto&& f) AFGPU_INLINE {
+    if constexpr (COLMAP)
+      with_pol(p.out_nt, f);
+    else
+      f(RtPol{p.out_nt});
+  };
+  with_out_pol([&](auto pol) AFGPU_INLINE {
+    if constexpr (EPI == EPI_R2C) {
       static_for<E>([&](auto mc) AFGPU_INLINE {
         constexpr int m = decltype(mc)::value;
-        const int k = t + m * TPT;
-        cx<T> s0 = v[m];
-        cx<T> x;
-        if (k == 0) {
-          x = {s0.x + s0.y, T(0)};
-          store(L, cx<T>{s0.x - s0.y, T(0)});
+        lat(t + m * TPT) = v[m];
+      });
+      __syncthreads();
+      if (active) {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          const int k = t + m * TPT;
+          cx<T> s0 = v[m];
+          cx<T> x;
+          if (k == 0) {
+            x = {s0.x + s0.y, T(0)};
+            store(pol, L, cx<T>{s0.x - s0.y, T(0)});
+          } else {
+            cx<T> s1c = conjg(lat(L - k));
+            cx<T> w = p.real_tw[k];
+            const T h = T(0.5);
+            cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
+            cx<T> d = s0 - s1c;
+            cx<T> bw = cmul(d, w);
+            cx<T> bb = {bw.x * h, bw.y * h};
+            x = a + mul_neg_i(bb);
+          }
+          store(pol, k, x);
+        });
+      }
+    } else {
+      if constexpr (EPI == EPI_TW4 && !STORE_T) {
+        // twiddle and store eight bins at a time: the scheduling barrier keeps
+        // the table loads of later groups from being hoisted, which would hold
+        // every twiddle in registers at once.  Group g takes registers
+        // m = g (mod E/8), which the last radix-32 stage's inner radix-8
+        // butterflies produce together, so their registers free early.
+        const unsigned n2 = p.tw4_blo ? (unsigned)blo : (unsigned)(col >> p.tw4_col_shift);
+        const unsigned mask = (1u << p.tw4_lo_bits) - 1u;
+        constexpr int GS = E < 8 ? E : 8;
+        static_for<E / GS>([&](auto gc) AFGPU_INLINE {
+          constexpr int g = decltype(gc)::value;
+          static_for<GS>([&](auto ic) AFGPU_INLINE {
+            constexpr int m = g + (E / GS) * decltype(ic)::value;
+            const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
+            const cx<T> w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
+            v[m] = cmul(v[m], w);
+            if (active) store(pol, t + m * TPT, v[m]);
+          });
+          __builtin_amdgcn_sched_barrier(0);
+        });
+        return;
+      }
+      if constexpr (EPI == EPI_TW4) {
+        const idx_t n2 = p.tw4_blo ? (idx_t)blo : (idx_t)(col >> p.tw4_col_shift);
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          const idx_t k1 = t + m * TPT;
+          v[m] = cmul(v[m], tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, n2 * k1));
+        });
+      }
+      if constexpr (STORE_T) {
+        static_assert(!COLMAP, "transposed store is for row-mapped passes");
+        const int c2 = tid % CT;
+        const int t2 = tid / CT;
+        const int colb = col0 + c2;
+        // re-aim the output at column colb (the thread's column changes here)
+        if constexpr (FAST) {
+          out0 = (unsigned)colb * (unsigned)p.out_cs * ES;
         } else {
-          cx<T> s1c = conjg(lat(L - k));
-          cx<T> w = p.real_tw[k];
-          const T h = T(0.5);
-          cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
-          cx<T> d = s0 - s1c;
-          cx<T> bw = cmul(d, w);
-          cx<T> bb = {bw.x * h, bw.y * h};
-          x = a + mul_neg_i(bb);
+          ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
+          out_re = p.out_re;
+          out_im = p.out_im;
         }
-        store(k, x);
-      });
-    }
-  } else {
-    if constexpr (EPI == EPI_TW4 && !STORE_T) {
-      // twiddle and store four bins at a time: the scheduling barrier keeps
-      // the table loads of later groups from being hoisted, which would hold
-      // every twiddle in registers at once
-      const unsigned n2 = p.tw4_blo ? (unsigned)blo : (unsigned)(col >> p.tw4_col_shift);
-      const unsigned mask = (1u << p.tw4_lo_bits) - 1u;
-      static_for<(E + 3) / 4>([&](auto gc) AFGPU_INLINE {
-        constexpr int g = decltype(gc)::value;
-        static_for<(E < 4 ? E : 4)>([&](auto ic) AFGPU_INLINE {
-          constexpr int m = 4 * g + decltype(ic)::value;
-          const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
-          const cx<T> w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
-          v[m] = cmul(v[m], w);
-          if (active) store(t + m * TPT, v[m]);
-        });
-        __builtin_amdgcn_sched_barrier(0);
-      });
-      return;
-    }
-    if constexpr (EPI == EPI_TW4) {
-      const idx_t n2 = p.tw4_blo ? (idx_t)blo : (idx_t)(col >> p.tw4_col_shift);
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        const idx_t k1 = t + m * TPT;
-        v[m] = cmul(v[m], tw4_lookup(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, n2 * k1));
-      });
-    }
-    if constexpr (STORE_T) {
-      static_assert(!COLMAP, "transposed store is for row-mapped passes");
-      const int c2 = tid % CT;
-      const int t2 = tid / CT;
-      const int colb = col0 + c2;
-      // re-aim the output at column colb (the thread's column changes here)
-      if constexpr (FAST) {
-        out0 = (unsigned)colb * (unsigned)p.out_cs * ES;
-      } else {
-        ob = (idx_t)bhi * p.out_bs + (idx_t)blo * p.out_bs2 + (idx_t)colb * p.out_cs;
-        out_re = p.out_re;
-        out_im = p.out_im;
-      }
-      if constexpr (HX) {
-        T xs[E];
-        static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
-        __syncthreads();
-        static_for<E>([&](auto mc) AFGPU_INLINE {
-          constexpr int m = decltype(mc)::value;
-          xs[m] = ldsf[c2 * LL + lds_pad(t2 + m * TPT)];
-        });
-        __syncthreads();
-        static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
-        __syncthreads();
-        if (colb < p.n_col) {
+        if constexpr (HX) {
+          T xs[E];
+          static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
+          __syncthreads();
           static_for<E>([&](auto mc) AFGPU_INLINE {
             constexpr int m = decltype(mc)::value;
-            const int k = t2 + m * TPT;
-            store(k, cx<T>{xs[m], ldsf[c2 * LL + lds_pad(k)]});
+            xs[m] = ldsf[c2 * LL + lds_pad(t2 + m * TPT)];
           });
-        }
-      } else {
-        static_for<E>([&](auto mc) AFGPU_INLINE {
-          constexpr int m = decltype(mc)::value;
-          lat(t + m * TPT) = v[m];
-        });
-        __syncthreads();
-        if (colb < p.n_col) {
+          __syncthreads();
+          static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
+          __syncthreads();
+          if (colb < p.n_col) {
+            static_for<E>([&](auto mc) AFGPU_INLINE {
+              constexpr int m = decltype(mc)::value;
+              const int k = t2 + m * TPT;
+              store(pol, k, cx<T>{xs[m], ldsf[c2 * LL + lds_pad(k)]});
+            });
+          }
+        } else {
           static_for<E>([&](auto mc) AFGPU_INLINE {
             constexpr int m = decltype(mc)::value;
-            const int k = t2 + m * TPT;
-            store(k, lds[c2 * LL + lds_pad(k)]);
+            lat(t + m * TPT) = v[m];
           });
+          __syncthreads();
+          if (colb < p.n_col) {
+            static_for<E>([&](auto mc) AFGPU_INLINE {
+              constexpr int m = decltype(mc)::value;
+              const int k = t2 + m * TPT;
+              store(pol, k, lds[c2 * LL + lds_pad(k)]);
+            });
+          }
         }
+      } else if (active) {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          store(pol, t + m * TPT, v[m]);
+        });
       }
-    } else if (active) {
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        store(t + m * TPT, v[m]);
-      });
     }
-  }
+  });
+}
+
+// LDS bytes of a pass tile (the shape fft_tile uses).
+template <typename T, int L, int CT, bool COLMAP, bool STORE_T, int SV, bool HX>
+constexpr size_t tile_lds_bytes() {
+  return PassShape<L, CT, SV, (!COLMAP && STORE_T) ? (HX ? (int)sizeof(T) : 2 * (int)sizeof(T)) : 0>::template lds_bytes<T, HX>();
 }
 
 // Minimum waves per SIMD the compiler must allow for: half-exchange tiles
 // that fit twice in a CU's 160 KiB of LDS are meant to run two groups per CU,
 // so their registers are bounded accordingly; everything else runs one.
-// LDS bytes of a pass tile (the shape fft_tile uses).
-template <typename T, int L, int CT, bool COLMAP, bool STORE_T, int SV, bool HX>
-constexpr size_t tile_lds_bytes() {
-  return PassShape<L, CT, SV, (!COLMAP && STORE_T && HX) ? 1 : 0>::template lds_bytes<T, HX>();
-}
 
 template <typename T, int L, int CT, bool COLMAP, bool STORE_T, int SV, bool HX>
 constexpr int pass_min_waves() {

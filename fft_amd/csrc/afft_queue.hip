@@ -332,10 +332,12 @@ constexpr size_t q_lds(int ct) {
 }
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr size_t kQLds = cmax(
-    cmax(cmax(q_lds<16>(QGroupCT<16>::value), q_lds<4096>(QGroupCT<4096>::value)),
-         cmax(q_lds<128>(QGroupCT<128>::value), q_lds<256>(QGroupCT<256>::value))),
-    cmax(cmax(q_lds<2>(QGroupCT<2>::value), q_lds<32>(QGroupCT<32>::value)),
-         cmax(q_lds<64>(QGroupCT<64>::value), q_lds<2048>(QGroupCT<2048>::value))));
+    cmax(cmax(cmax(q_lds<16>(QGroupCT<16>::value), q_lds<4096>(QGroupCT<4096>::value)),
+              cmax(q_lds<128>(QGroupCT<128>::value), q_lds<256>(QGroupCT<256>::value))),
+         cmax(cmax(q_lds<2>(QGroupCT<2>::value), q_lds<32>(QGroupCT<32>::value)),
+              cmax(q_lds<64>(QGroupCT<64>::value), q_lds<2048>(QGroupCT<2048>::value)))),
+    // pass-B tiles of the chains: transposed store, padded rows
+    tile_lds_bytes<float, kBigN2, 16, false, true, 0, false>());
 
 int q_group_ct(int lg) {
   switch (lg) {
