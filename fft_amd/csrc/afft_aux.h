@@ -44,6 +44,30 @@ struct RealTrArgs {
   const cx<T>* real_tw;  // W_{2h}^k, k < h
 };
 
+// Pass B of the four-step 1-D r2c fused with the real post-pass and the
+// four-step transposition (afft_realpair.hip): the input is the pass-A
+// output Y[b][k1][n2'] (n1 rows of n2, interleaved); each work-group runs the
+// n2-point FFTs of 2R rows k1 and n1 - k1, pairs bin k2 of row k1 with bin
+// n2 - 1 - k2 of row n1 - k1 (Z[k] with Z[h - k], k = k1 + n1*k2) and writes
+// the h+1 bins of the r2c in natural order.
+template <typename T>
+struct RealPairArgs {
+  const cx<T>* in;
+  T* out_re;
+  T* out_im;
+  int out_il;
+  int out_nt;            // non-temporal stores
+  idx_t in_bs, out_bs;   // batch strides (complex / output elements)
+  int n1, n2;
+  idx_t nbatch;
+  const cx<T>* stage_tw; // stage twiddles of the n2-point rows (radix-32 schedule)
+  const cx<T>* real_tw;  // W_{2h}^k, k < h
+  const cx<T>* row_tw;   // W_{2 n2}^k, k < n2  (= W_{2h}^(n1 k))
+};
+
+// Whether real_pair has a kernel for rows of n2 points and n1 rows.
+bool real_pair_supported(int elem_bytes, int n1, int n2);
+
 template <typename T>
 hipError_t gen_stage_tw(cx<T>* out, const StageList& sl, hipStream_t st);
 template <typename T>
@@ -54,6 +78,8 @@ template <typename T>
 hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st);
 template <typename T>
 hipError_t real_tr(const RealTrArgs<T>& a, hipStream_t st);
+template <typename T>
+hipError_t real_pair(const RealPairArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

@@ -159,33 +159,65 @@ hipError_t real_rows(const RealRowsArgs<T>& a, bool inverse, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Transposing forward real pass.  A work-group takes a block of 32 x 32
-// (k1, k2) entries: it reads Y[k1][k2] and the partner Z[h - k] (row n1 - k1,
-// column n2 - 1 - k2; row 0: column (n2 - k2) mod n2), both as 256-byte row
-// segments, forms X[k] = (Z[k] + conj Z[h-k])/2 - i (Z[k] - conj Z[h-k]) W_2h^k
+// Transposing forward real pass.  A work-group takes a block of 32 x 128
+// (k1, k2) entries (32 x 64 in FP64): it reads Y[k1][k2] and the partner
+// Z[h - k] (row n1 - k1, column n2 - 1 - k2; row 0: column (n2 - k2) mod n2),
+// both as 1 KiB row segments, forms X[k] = (Z[k] + conj Z[h-k])/2 - i (Z[k] - conj Z[h-k]) W_2h^k
 // / 2 (onedim.hpp:1066-1106 arithmetic, W_2h^k = W_2h^k1 * W_2h^(n1 k2)), and
 // writes the bins k1 + n1*k2 through LDS as 256-byte runs of consecutive k1.
-// Bin 0 and bin h come from Z[0] (its block writes both).
+// Bin 0 and bin h come from Z[0] (its block writes both).  Each thread holds
+// 16 (FP64: 8) entries, so a work-group keeps 32 KiB of loads in flight.
+//
+// Block order.  The partner entries of tile t = i1 + nb1 * i2 (nb1 x nb2 tiles
+// per transform) lie in its mirror tile T - 1 - t (rows n1 - k1 sit one row
+// off the tile grid, so 31 of its 32 rows).  Work-groups are dispatched
+// round-robin over the 8 XCDs, block B on XCD B % 8; blocks 16 j + x and
+// 16 j + 8 + x run tile pair 8 j + x on the same XCD one slot apart, so the
+// second of the two reads of every entry hits that XCD's L2.  The grid is
+// rounded up to whole groups of 16; blocks past the last pair exit.
+constexpr int kTrRows = 32;
+template <typename T>
+constexpr int tr_cols() {
+  return sizeof(T) == 4 ? 128 : 64;  // 33 KiB of LDS either way
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) real_tr_kernel(RealTrArgs<T> a) {
-  constexpr int TB = 32;
-  __shared__ cx<T> sx[TB][TB + 1];
-  const int nb1 = a.n1 / TB, nb2 = a.n2 / TB;
-  int bid = blockIdx.x;
-  const idx_t b = bid / (nb1 * nb2);
-  bid -= (int)(b * (nb1 * nb2));
-  const int a0 = (bid % nb1) * TB, c0 = (bid / nb1) * TB;
+  constexpr int TB = kTrRows, TC = tr_cols<T>(), PER = TB * TC / 256;
+  __shared__ cx<T> sx[TC][TB + 1];
+  const int nb1 = a.n1 / TB, nb2 = a.n2 / TC;
+  const int ntile = nb1 * nb2;
+  idx_t b;
+  int tile;
+  if (ntile >= 2) {
+    const idx_t B = blockIdx.x;
+    const idx_t pair = (B >> 4) * 8 + (B & 7);
+    const int half = ntile / 2;
+    b = pair / half;
+    if (b >= a.nbatch) return;
+    const int pp = (int)(pair - b * half);
+    tile = (B & 8) ? ntile - 1 - pp : pp;
+  } else {
+    b = blockIdx.x;
+    tile = 0;
+  }
+  const int a0 = (tile % nb1) * TB, c0 = (tile / nb1) * TC;
   const cx<T>* Y = a.in + b * a.in_bs;
   const idx_t h = a.h;
+  // this thread's column is fixed (256 is a multiple of TC): one twiddle
+  // factor W_2h^(n1 k2) and one partner column per thread
+  const int q = threadIdx.x % TC;
+  const int k2 = c0 + q;
+  const cx<T> w2 = a.real_tw[(idx_t)a.n1 * k2];
+  const int p2n = a.n2 - 1 - k2;              // partner column, rows k1 > 0
+  const int p20 = (a.n2 - k2) & (a.n2 - 1);   // partner column, row 0
 #pragma unroll
-  for (int i = 0; i < TB * TB / 256; i++) {
-    const int e = threadIdx.x + 256 * i;
-    const int r = e / TB, q = e % TB;
-    const int k1 = a0 + r, k2 = c0 + q;
+  for (int i = 0; i < PER; i++) {
+    const int r = threadIdx.x / TC + (256 / TC) * i;
+    const int k1 = a0 + r;
     const cx<T> z = Y[(idx_t)k1 * a.n2 + k2];
     const int p1 = k1 == 0 ? 0 : a.n1 - k1;
-    const int p2 = k1 == 0 ? ((a.n2 - k2) & (a.n2 - 1)) : a.n2 - 1 - k2;
-    const cx<T> zp = Y[(idx_t)p1 * a.n2 + p2];
+    const cx<T> zp = Y[(idx_t)p1 * a.n2 + (k1 == 0 ? p20 : p2n)];
     cx<T> x;
     if (k1 == 0 && k2 == 0) {
       x = cx<T>{z.x + z.y, T(0)};
@@ -198,7 +230,7 @@ __global__ void __launch_bounds__(256) real_tr_kernel(RealTrArgs<T> a) {
         a.out_im[o] = nyq.y;
       }
     } else {
-      const cx<T> w = cmul(a.real_tw[k1], a.real_tw[(idx_t)a.n1 * k2]);
+      const cx<T> w = cmul(a.real_tw[k1], w2);
       const cx<T> s1c = conjg(zp);
       const T hf = T(0.5);
       const cx<T> aa = {(z.x + s1c.x) * hf, (z.y + s1c.y) * hf};
@@ -209,12 +241,13 @@ __global__ void __launch_bounds__(256) real_tr_kernel(RealTrArgs<T> a) {
     sx[q][r] = x;
   }
   __syncthreads();
+  const idx_t ob = b * a.out_bs + a0 + (idx_t)a.n1 * c0;
 #pragma unroll
-  for (int i = 0; i < TB * TB / 256; i++) {
+  for (int i = 0; i < PER; i++) {
     const int e = threadIdx.x + 256 * i;
-    const int r = e % TB, q = e / TB;
-    const idx_t o = b * a.out_bs + (idx_t)(a0 + r) + (idx_t)a.n1 * (c0 + q);
-    const cx<T> x = sx[q][r];
+    const int r = e % TB, qq = e / TB;
+    const idx_t o = ob + r + (idx_t)a.n1 * qq;
+    const cx<T> x = sx[qq][r];
     if (a.out_il) {
       reinterpret_cast<cx<T>*>(a.out_re)[o] = x;
     } else {
@@ -226,8 +259,12 @@ __global__ void __launch_bounds__(256) real_tr_kernel(RealTrArgs<T> a) {
 
 template <typename T>
 hipError_t real_tr(const RealTrArgs<T>& a, hipStream_t st) {
-  if (a.n1 % 32 || a.n2 % 32 || a.n1 < 32 || a.n2 < 32) return hipErrorInvalidValue;
-  const idx_t blocks = a.nbatch * (idx_t)(a.n1 / 32) * (a.n2 / 32);
+  constexpr int TC = tr_cols<T>();
+  if (a.n1 % kTrRows || a.n2 % TC || a.n1 < kTrRows || a.n2 < TC) return hipErrorInvalidValue;
+  const idx_t ntile = (idx_t)(a.n1 / kTrRows) * (a.n2 / TC);
+  if (ntile > 1 && (ntile & 1)) return hipErrorInvalidValue;
+  // tile pairs in groups of 8 (16 blocks), one block per tile when ntile = 1
+  const idx_t blocks = ntile >= 2 ? (a.nbatch * (ntile / 2) + 7) / 8 * 16 : a.nbatch;
   if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL(real_tr_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, st, a);

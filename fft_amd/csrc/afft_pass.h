@@ -215,14 +215,14 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
 
 // Row length (in LDS entries of EB bytes) of a transposed-store tile: the
 // store phase reads entry c2 * LL + k with c2 = lane % CT and k running over
-// lane / CT, so LL is padded to LL = 64 / CT modulo the entries per bank sweep
-// (128 / EB), which spreads the 64 lanes evenly over the 32 four-byte LDS
-// banks (the minimum number of cycles for a wave).
+// lane / CT.  The LDS serves mod = 128 / EB lanes per cycle (32 four-byte
+// banks), so LL = mod / CT (mod mod) puts the CT rows of those lanes mod / CT
+// banks apart and the mod / CT values of k in between: no bank is hit twice.
 template <int L, int CT, int EB>
 constexpr int transposed_row_len() {
   constexpr int base = LdsLen<L>::value;
   constexpr int mod = EB > 0 ? 128 / EB : 1;
-  constexpr int want = (64 / CT) % mod;
+  constexpr int want = CT <= mod ? (mod / CT) % mod : 1;
   return base + ((want - base % mod) % mod + mod) % mod;
 }
 
@@ -292,12 +292,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
 
-// One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
-// tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
 // A cache policy known only at run time (one branch per access).
 struct RtPol {
   int nt;
 };
+
+// One tile (CT columns) of a pass, executed by the NT = CT * L/E threads
+// tid = 0..NT-1 of a work-group, with `lds` holding CT * LdsLen<L> entries.
 
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {

@@ -69,7 +69,7 @@ constexpr int kMaxDim = 64;
 constexpr int kMaxPass = 160;
 
 enum Role { R_SRC = 0, R_DST = 1, R_SCR = 2, R_SCR2 = 3 };
-enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3 };
+enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3, PT_REALPAIR = 4 };
 
 struct Side {
   int role = R_SRC;
@@ -119,6 +119,7 @@ struct Variant {
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
   int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
+  int real_pair = 0;  // long 1-D r2c: 1 = pass B fused with the real post-pass where compiled, 0 (default) / -1 = not
 };
 
 struct Plan {
@@ -509,7 +510,6 @@ struct Builder {
       return;
     }
     need_scratch(R_SCR, h);
-    need_scratch(R_SCR2, h);
     const int bits = (ilog2(h) + 1) / 2;
     const void* tw4 = get_table<T>(1, h, bits);
     if (!tw4) {
@@ -531,6 +531,32 @@ struct Builder {
     a->tw4_hi = tw4;
     a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
     a->tw4_bits = bits;
+    if (P.var.real_pair > 0 && real_pair_supported((int)sizeof(T), (int)n1, (int)n2)) {
+      // rows k1 and n1 - k1 transformed together, the real post-pass applied
+      // between them and the bins stored in natural order (afft_realpair.hip)
+      PassRec* r = add();
+      if (!r) return;
+      r->type = PT_REALPAIR;
+      r->L = (int)n2;
+      r->CT = 16;
+      r->sv = 1;
+      r->hx = 1;
+      scratch_side(r->in, R_SCR);
+      r->in.s_j = 1;
+      r->in.s_c = n2;
+      r->in.s_b = h;
+      r->out = out;
+      r->out.s_j = 1;
+      r->out.s_b = out.s_u;
+      r->rr_h = h;
+      r->rr_m = n1;
+      r->stage_tw = get_table<T>(0, (int)n2, 1);
+      r->real_tw = get_table<T>(2, h, 0);
+      r->tw4_hi = get_table<T>(2, n2, 0);
+      if (!r->stage_tw || !r->real_tw || !r->tw4_hi) ok = false;
+      return;
+    }
+    need_scratch(R_SCR2, h);
     PassRec* b = add();
     if (!b || !fft_pass(b, (int)n2, 0, EPI_PLAIN, 0)) return;
     scratch_side(b->in, R_SCR);
@@ -1096,6 +1122,24 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     HIP_OK(real_rows<T>(a, r.rr_inv != 0, st));
     return true;
   }
+  if (r.type == PT_REALPAIR) {
+    RealPairArgs<T> a{};
+    a.in = reinterpret_cast<const cx<T>*>(ire);
+    a.out_re = ore;
+    a.out_im = oim;
+    a.out_il = r.out.il;
+    a.out_nt = r.out_nt;
+    a.in_bs = r.in.s_b;
+    a.out_bs = r.out.s_b;
+    a.n1 = (int)r.rr_m;
+    a.n2 = r.L;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    a.row_tw = (const cx<T>*)r.tw4_hi;
+    HIP_OK(real_pair<T>(a, st));
+    return true;
+  }
   if (r.type == PT_REALTR) {
     RealTrArgs<T> a{};
     a.in = reinterpret_cast<const cx<T>*>(ire);
@@ -1346,7 +1390,7 @@ TuneCache& tcache() {
 }
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
-//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline>]]
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline> [<real_pair>]]]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1358,8 +1402,8 @@ void tcache_load_locked(TuneCache& c) {
   while (fgets(line, sizeof(line), f)) {
     Variant v;
     long long ch = 0;
-    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
-                           &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline);
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
+                           &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline, &v.real_pair);
     if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
@@ -1386,8 +1430,8 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
-          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline);
+  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
+          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair);
   fclose(f);
 }
 
@@ -1430,6 +1474,14 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
               if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
               cands.push_back(v);
             }
+  // long 1-D r2c plans also try the other form of the real pass
+  for (int i = 0; i < P.npass; i++)
+    if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR) {
+      Variant v;
+      v.real_pair = P.pass[i].type == PT_REALPAIR ? -1 : 1;
+      cands.push_back(v);
+      break;
+    }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
   float best = 1e30f;
@@ -1645,6 +1697,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.sched = v->sched;
   P.var.half_lds = v->half_lds;
   P.var.pipeline = v->pipeline;
+  P.var.real_pair = v->real_pair;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1694,7 +1747,7 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->sched = P.var.sched;
   v->half_lds = P.var.half_lds;
   v->pipeline = P.var.pipeline;
-  v->pad_ = 0;
+  v->real_pair = P.var.real_pair;
   return 0;
 }
 
@@ -1739,6 +1792,9 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALTR)
     snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
+  else if (r.type == PT_REALPAIR)
+    snprintf(buf, len, "real_pair L=%d rows=%d h=%lld n1=%lld out(role=%d il=%d)", r.L, r.CT, (long long)r.rr_h,
+             (long long)r.rr_m, r.out.role, r.out.il);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
   else

@@ -1,0 +1,240 @@
+// afft_realpair.hip -- pass B of the four-step 1-D r2c with the real post-pass
+// and the output transposition fused in.
+//
+// A long r2c (N reals, h = N/2 = n1 * n2 complex points) runs its h-point c2c
+// as a four-step: pass A (n1-point columns with the W_h twiddle) leaves
+// Y[k1][n2'] in scratch, and the n2-point FFT of row k1 gives
+// Z[k1 + n1*k2] at bin k2.  The real post-pass (onedim.hpp:1066-1106,
+// inverse=false) forms
+//     X[k] = (Z[k] + conj Z[h-k]) / 2 - i W_2h^k (Z[k] - conj Z[h-k]) / 2,
+// and h - k = (n1 - k1) + n1*(n2 - 1 - k2) for k1 > 0, n1*((n2 - k2) mod n2)
+// for k1 = 0.  So the partner of every bin of row k1 is a bin of row n1 - k1:
+// a work-group that transforms rows k1 and n1 - k1 together has both members
+// of every pair.  It holds 2R rows:
+//   slots c <  R: rows j*R + c,
+//   slots c >= R: rows n1 - j*R - (c - R)   (tile 0: slot R is row n1/2),
+// so slot c pairs with slot c +- R, and rows 0 and n1/2 (tile 0, slots 0 and
+// R) pair with themselves.  After the row FFTs the partner values cross the
+// LDS (real parts, then imaginary parts), each thread forms X for its bins,
+// and a second exchange turns the rows around so that 2R lanes store R
+// consecutive bins of each row set (bins k1 + n1*k2 are contiguous in k1).
+// Bin 0 of row 0 gives X[0] = Re Z[0] + Im Z[0] through the same formula
+// (W^0 = 1, partner itself); its thread also writes X[h] = Re Z[0] - Im Z[0].
+//
+// This replaces a row pass into a second scratch plus real_tr_kernel: the
+// intermediate Z never goes back to memory, and the plan needs half the
+// scratch.  It is a variant the caller asks for (afft_gpu_variant.real_pair):
+// a work-group of 16 rows holds 64 registers of data per lane plus the
+// partner values, is bounded to 128 VGPRs (two groups per CU for 1024-point
+// rows, one for 2048) and spills; measured, it runs 5-10 % behind the two
+// kernels it replaces for 1024-point rows and 2.5x behind for 2048.
+#include <climits>
+
+#include "afft_aux.h"
+#include "afft_pass.h"
+
+namespace afgpu {
+
+// Row held by slot c of tile j (see above).
+__device__ __forceinline__ int rp_row(int j, int c, int R, int n1) {
+  if (c < R) return j * R + c;
+  const int c2 = c - R;
+  if (j == 0 && c2 == 0) return n1 / 2;
+  return n1 - j * R - c2;
+}
+
+template <typename T, int L, int R, int SV>
+constexpr size_t rp_lds_bytes() {
+  // one real per entry (half exchange), rows padded for the transposed reads
+  return PassShape<L, 2 * R, SV, (int)sizeof(T)>::template lds_bytes<T, true>();
+}
+
+template <typename T, int L, int R, int SV>
+constexpr int rp_min_waves() {
+  constexpr int nt = PassShape<L, 2 * R, SV>::NT;
+  constexpr int groups = 2 * rp_lds_bytes<T, L, R, SV>() <= 160 * 1024 ? 2 : 1;
+  return groups * nt / 256 > 1 ? groups * nt / 256 : 1;
+}
+
+template <typename T, int L, int R, int SV>
+__global__ void __launch_bounds__((PassShape<L, 2 * R, SV>::NT), (rp_min_waves<T, L, R, SV>()))
+    real_pair_kernel(RealPairArgs<T> a) {
+  static_assert(sizeof(T) == 4, "FP32 rows (the split store writes 4-byte reals)");
+  using PS = PassShape<L, 2 * R, SV, (int)sizeof(T)>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  constexpr int LL = PS::LL;
+  constexpr int CT = 2 * R;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+
+  const int tiles = a.n1 / CT;
+  const idx_t b = blockIdx.x / tiles;
+  const int j = (int)(blockIdx.x - b * tiles);
+  const int tid = threadIdx.x;
+  const int c = tid / TPT, t = tid % TPT;
+  const int row = rp_row(j, c, R, a.n1);
+  const bool self = j == 0 && (c == 0 || c == R);
+  const bool row0 = j == 0 && c == 0;
+  const int pc = self ? c : (c < R ? c + R : c - R);
+
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
+  // entries t + m*TPT of this row: one base, constant steps (TPT is a multiple
+  // of 16, so lds_pad(t + m*TPT) = lds_pad(t) + m*MS); written this way the
+  // LDS accesses take immediate offsets instead of one address register each
+  constexpr int MS = TPT + TPT / 16;
+  T* __restrict__ own = ldsf + c * LL + lds_pad(t);
+  // LDS entry of the partner of bin k = t + m*TPT: bin L-1-k of the partner
+  // row, or (L-k) mod L for row 0.  Both are base - m*TPT with one base per
+  // thread (bin 0 of row 0, its own partner, aside); TPT is a multiple of 16,
+  // so the padded index steps by a constant and the LDS reads take immediate
+  // offsets.
+  static_assert(TPT % 16 == 0, "padded partner index must be affine in m");
+  const int pbase = row0 ? L - t : L - 1 - t;
+  const int pent = pc * LL + lds_pad(pbase);
+  const int pent0 = (row0 && t == 0) ? pc * LL : pent;
+  auto partner = [&](auto mc) -> T {
+    constexpr int m = decltype(mc)::value;
+    if constexpr (m == 0)
+      return ldsf[pent0];
+    else
+      return ldsf[pent - m * MS];
+  };
+
+  // rows in (buffer addressing: one descriptor per transform, 32-bit byte
+  // offsets), FFT along each row
+  constexpr unsigned ES = sizeof(cx<T>);
+  const unsigned h = (unsigned)a.n1 * (unsigned)L;
+  const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + b * a.in_bs, h * ES);
+  const unsigned rowoff = (unsigned)row * (unsigned)L * ES + (unsigned)t * ES;
+  cx<T> v[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    v[m] = buf_ld<T, 0>(rin, rowoff + (unsigned)(m * TPT) * ES);
+  });
+  run_stages<T, L, SV, 0, true>(v, t, latf, a.stage_tw);
+
+  // real post-pass: partner real parts, then imaginary parts, through the LDS
+  const cx<T> wr = a.real_tw[row];
+  T pre[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE { own[decltype(mc)::value * MS] = v[decltype(mc)::value].x; });
+  __syncthreads();
+  static_for<E>([&](auto mc) AFGPU_INLINE { pre[decltype(mc)::value] = partner(mc); });
+  __syncthreads();
+  static_for<E>([&](auto mc) AFGPU_INLINE { own[decltype(mc)::value * MS] = v[decltype(mc)::value].y; });
+  __syncthreads();
+  const T nyq = v[0].x - v[0].y;  // used by row 0, bin 0 only
+  // eight bins at a time: the scheduling barrier keeps the twiddle loads of
+  // later groups from being hoisted (all of them would not fit in registers)
+  static_for<E / 8>([&](auto gc) AFGPU_INLINE {
+    static_for<8>([&](auto ic) AFGPU_INLINE {
+      constexpr int m = 8 * decltype(gc)::value + decltype(ic)::value;
+      const int k = t + m * TPT;
+      const cx<T> z = v[m];
+      const cx<T> zc{pre[m], -partner(std::integral_constant<int, m>{})};  // conj Z[h - k]
+      const cx<T> w = cmul(wr, a.row_tw[k]);
+      const T hf = T(0.5);
+      const cx<T> s = {(z.x + zc.x) * hf, (z.y + zc.y) * hf};
+      const cx<T> dw = cmul(z - zc, w);
+      v[m] = s + mul_neg_i(cx<T>{dw.x * hf, dw.y * hf});
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  // pin the results here: without it the compiler sinks the arithmetic above
+  // past the barrier, where its inputs then compete with the store phase for
+  // registers
+#pragma unroll
+  for (int m = 0; m < E; m++) asm volatile("" : "+v"(v[m].x), "+v"(v[m].y));
+  __syncthreads();
+
+  // transpose through the LDS: lane c2 of each group of 2R lanes stores row
+  // slot c2, so R lanes write R consecutive bins of one row set
+  const int c2 = tid % CT, t2 = tid / CT;
+  const int orow = rp_row(j, c2, R, a.n1);
+  T xs[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE { own[decltype(mc)::value * MS] = v[decltype(mc)::value].x; });
+  __syncthreads();
+  const T* __restrict__ col = ldsf + c2 * LL + lds_pad(t2);
+  static_for<E>([&](auto mc) AFGPU_INLINE { xs[decltype(mc)::value] = col[decltype(mc)::value * MS]; });
+  __syncthreads();
+  static_for<E>([&](auto mc) AFGPU_INLINE { own[decltype(mc)::value * MS] = v[decltype(mc)::value].y; });
+  __syncthreads();
+  // bins 0..h of this transform: interleaved (one 8-byte store per bin) or
+  // split (a 4-byte store into each of the re and im arrays)
+  const idx_t ob = b * a.out_bs;
+  const bool il = a.out_il != 0;
+  const unsigned obytes = (h + 1) * (il ? ES : (unsigned)sizeof(T));
+  const __amdgpu_buffer_rsrc_t rre =
+      make_rsrc(il ? (void*)(reinterpret_cast<cx<T>*>(a.out_re) + ob) : (void*)(a.out_re + ob), obytes);
+  const __amdgpu_buffer_rsrc_t rim = make_rsrc(il ? (void*)a.out_re : (void*)(a.out_im + ob), il ? 0u : obytes);
+  // bin orow + n1*(t2 + m*TPT): the per-lane part in the vector offset, the
+  // per-m part (uniform) in the scalar offset
+  const unsigned esz = il ? ES : (unsigned)sizeof(T);
+  const unsigned vbin = (unsigned)orow + (unsigned)a.n1 * (unsigned)t2;
+  const unsigned mstep = (unsigned)a.n1 * (unsigned)TPT * esz;
+  auto put = [&](auto polc, unsigned voff, unsigned soff, cx<T> x) AFGPU_INLINE {
+    constexpr int POL = decltype(polc)::value;
+    if (il) {
+      const u32x2 w = {__float_as_uint(x.x), __float_as_uint(x.y)};
+      __builtin_amdgcn_raw_buffer_store_b64(w, rre, voff, soff, POL);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x.x), rre, voff, soff, POL);
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x.y), rim, voff, soff, POL);
+    }
+  };
+  auto store_all = [&](auto polc) AFGPU_INLINE {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const cx<T> x{xs[m], col[m * MS]};
+      put(polc, vbin * esz, (unsigned)m * mstep, x);
+    });
+    if (row0 && t == 0) put(polc, h * esz, 0u, cx<T>{nyq, T(0)});
+  };
+  if (a.out_nt)
+    store_all(std::integral_constant<int, 2>{});
+  else
+    store_all(std::integral_constant<int, 0>{});
+}
+
+namespace {
+constexpr int kRpR = 8;  // row pairs per work-group: 8 consecutive bins per store run
+
+template <typename T, int L>
+hipError_t launch_rp(const RealPairArgs<T>& a, idx_t blocks, hipStream_t st) {
+  constexpr int SV = 1;
+  constexpr size_t lds = rp_lds_bytes<T, L, kRpR, SV>();
+  auto k = real_pair_kernel<T, L, kRpR, SV>;
+  if (lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(PassShape<L, 2 * kRpR, SV>::NT), lds, st, a);
+  return hipGetLastError();
+}
+}  // namespace
+
+bool real_pair_supported(int elem_bytes, int n1, int n2) {
+  return elem_bytes == 4 && (n2 == 1024 || n2 == 2048) && n1 >= 2 * kRpR && n1 % (2 * kRpR) == 0;
+}
+
+template <typename T>
+hipError_t real_pair(const RealPairArgs<T>& a, hipStream_t st) {
+  if (!real_pair_supported((int)sizeof(T), a.n1, a.n2)) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (idx_t)(a.n1 / (2 * kRpR));
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > INT_MAX) return hipErrorInvalidValue;
+  if constexpr (sizeof(T) == 4) {
+    if (a.n2 == 1024) return launch_rp<T, 1024>(a, blocks, st);
+    return launch_rp<T, 2048>(a, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template hipError_t real_pair<float>(const RealPairArgs<float>&, hipStream_t);
+template hipError_t real_pair<double>(const RealPairArgs<double>&, hipStream_t);
+
+}  // namespace afgpu

@@ -260,9 +260,61 @@ def _real_tr(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
 
 
-def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32):
-    """Executes the plan model; returns the dst arrays (float64)."""
+def _rp_row(j, c, R, n1):
+    """Row held by slot c of tile j of real_pair_kernel (afft_realpair.hip)."""
+    if c < R:
+        return j * R + c
+    c2 = c - R
+    if j == 0 and c2 == 0:
+        return n1 // 2
+    return n1 - j * R - c2
+
+
+def _real_pair(p, bufs, start, cnt, pi):
+    """Pass B of the four-step r2c fused with the real pass (afft_realpair.hip):
+    tile j of each transform transforms rows _rp_row(j, c) (c < 2R) of
+    Y[b][k1][:] (n2 points each), pairs slot c with slot c +- R (rows 0 and
+    n1/2 with themselves) and writes bins row + n1*k2, plus bin h."""
+    h, n1, n2 = p.rr_h, p.rr_m, p.L
+    R = p.CT // 2
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    tiles = n1 // p.CT
+    covered = np.zeros((cnt, n1), dtype=np.int64)
+    for b in range(cnt):
+        for j in range(tiles):
+            rows = [_rp_row(j, c, R, n1) for c in range(p.CT)]
+            covered[b, rows] += 1
+            e_in = (uin + b * p.in_.s_b + np.array(rows).reshape(-1, 1) * p.in_.s_c
+                    + np.arange(n2).reshape(1, -1) * p.in_.s_j)
+            Z = np.fft.fft(_gather(p.in_, bufs, e_in, f"pass {pi} load"), axis=-1)
+            k2 = np.arange(n2)
+            X = np.empty_like(Z)
+            for c in range(p.CT):
+                row = rows[c]
+                self_pair = j == 0 and c in (0, R)
+                pc = c if self_pair else (c + R if c < R else c - R)
+                mirror = ((n2 - k2) % n2) if (j == 0 and c == 0) else (n2 - 1 - k2)
+                zp = Z[pc, mirror]
+                k = row + n1 * k2
+                w = np.exp(-2j * np.pi * k / (2 * h))
+                X[c] = 0.5 * (Z[c] + np.conj(zp)) - 1j * 0.5 * (Z[c] - np.conj(zp)) * w
+            bins = np.array(rows).reshape(-1, 1) + n1 * k2.reshape(1, -1)
+            _scatter(p.out, bufs, uout + b * p.out.s_b + bins * p.out.s_j, X, f"pass {pi} store")
+            if j == 0:
+                z0 = Z[0, 0]
+                _scatter(p.out, bufs, np.array([uout + b * p.out.s_b + h * p.out.s_j]),
+                         np.array([z0.real - z0.imag + 0j]), f"pass {pi} store")
+    if not np.all(covered == 1):
+        raise AssertionError(f"pass {pi}: rows not covered exactly once")
+
+
+def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
+    """Executes the plan model; returns the dst arrays (float64).  edit
+    (optional) maps the inspected pass list to the one executed."""
     info, passes = inspect(kind, dims, fmt, batch, prec)
+    if edit is not None:
+        passes = edit(passes)
     bufs = Buffers(kind, dims, fmt, batch, info)
     if bufs.arrays["src_re"].size != np.asarray(src_re).size:
         raise ValueError("src_re size does not match the API contract")
@@ -278,6 +330,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32):
                 _real_rows(p, bufs, start, cnt, pi)
             elif p.type == 3:
                 _real_tr(p, bufs, start, cnt, pi)
+            elif p.type == 4:
+                _real_pair(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

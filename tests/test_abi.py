@@ -98,7 +98,23 @@ def _unpack(re_, im_, fmt):
     return re_[0::2] + 1j * re_[1::2] if fmt == IL else re_ + 1j * im_
 
 
-def model_check(kind, dims, fmt, batch, prec=32):
+def fuse_real_pass(passes):
+    """The pass list of the fused-real-pass variant (afft_gpu_variant.real_pair
+    = 1) from the default one: the row pass B (type 0) and the transposing
+    real pass (type 3) become one type-4 pass with B's input and the real
+    pass's output, 16 rows (8 pairs) per work-group."""
+    assert [p.type for p in passes[-2:]] == [0, 3], [p.type for p in passes]
+    b, tr = passes[-2], passes[-1]
+    f = M.PassInfo()
+    ctypes.pointer(f)[0] = tr
+    f.type = 4
+    f.L = b.L
+    f.CT = 16
+    f.in_ = b.in_
+    return passes[:-2] + [f]
+
+
+def model_check(kind, dims, fmt, batch, prec=32, edit=None):
     s = O.squeeze_shape(dims)
     n = int(np.prod(s))
     if kind in ("c", "ci"):
@@ -109,7 +125,7 @@ def model_check(kind, dims, fmt, batch, prec=32):
         ref = np.stack([O.fft(x[b].reshape(s), inverse=(kind == "ci")).ravel() for b in range(batch)])
     elif kind == "r":
         x = O.normal(2, batch * n).astype(np.float64).reshape(batch, n)
-        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, x, None, prec)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, x, None, prec, edit)
         y = _unpack(dre, dim_, fmt).reshape(batch, -1)
         ref = np.stack([O.rfft(x[b].reshape(s)).ravel() for b in range(batch)])
     else:
@@ -135,6 +151,16 @@ def test_plan_model_matches_oracle(kind, dims):
         for batch in (1, 3):
             err, info, passes = model_check(kind, dims, fmt, batch)
             assert err < 1e-9, (kind, dims, fmt, batch, err)
+
+
+@pytest.mark.parametrize("lg", [16, 17, 20, 22])
+def test_plan_model_fused_real_pass(lg):
+    # the opt-in fused real pass (rows k1 and n1 - k1 per work-group, 1024- or
+    # 2048-point rows): slot mapping, partner bins, row 0 / row n1/2 and the
+    # Nyquist bin against the oracle, both formats, a batch of 3
+    for fmt in (IL, SP):
+        err, info, passes = model_check("r", (1 << lg,), fmt, 3 if lg < 22 else 1, edit=fuse_real_pass)
+        assert err < 1e-9, (lg, fmt, err)
 
 
 def test_plan_model_fp64_and_chunking():

@@ -228,13 +228,15 @@ def test_errors_leave_state_clean(G):
     assert "power of two" in _lib.last_error()
 
 
-VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chunk=ch)
-            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8) for pl, ch in ((1, 1), (0, 0))]
+VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chunk=ch, real_pair=rp)
+            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8) for pl, ch in ((1, 1), (0, 0))
+            for rp in (1, -1)]
 
 
 @pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
                                           ("c", 1 << 17, 2), ("c", 1 << 14, 3), ("ci", 1 << 13, 3),
-                                          ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 1)])
+                                          ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 3),
+                                          ("r", 1 << 17, 3)])
 def test_kernel_variants_against_oracle(G, kind, n, batch):
     """Every kernel-shape variant (radix-32 schedules, half-LDS exchange,
     narrow tiles) against the FP64 oracle; the variants only change speed."""
@@ -281,6 +283,10 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
             assert O.rel_l2(yb, ref[b]) <= O.bound(n), (kind, n, v, b)
     if kind in ("c", "ci") and n > (1 << 14):
         assert len(seen) >= 2, seen  # four-step plans do have alternatives
+    if kind == "r" and n >= (1 << 16):
+        # the fused real pass and the row pass + transposing real pass
+        assert any("real_pair" in d for s in seen for d in s), seen
+        assert any("real_tr" in d for s in seen for d in s), seen
     p.close()
 
 
