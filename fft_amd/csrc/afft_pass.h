@@ -141,11 +141,25 @@ __device__ __forceinline__ cx<T> tw4_lookup32(const cx<T>* __restrict__ hi, cons
 template <typename T, int L, int SV>
 using Regs = cx<T>[Sched<L, SV>::E];
 
+// Step, in padded LDS entries, between index i and i + k*d when d is a
+// multiple of 16 (lds_pad(i + 16q) = lds_pad(i) + 17q): 0 when d is not.
+template <int D>
+constexpr int pad_step() {
+  return D % 16 == 0 ? D + D / 16 : 0;
+}
+
 // Stage S and the ones after it.  HX (half exchange): `lat` addresses one real
 // per entry and each exchange between stages moves the real parts, then the
 // imaginary parts, through the same LDS words -- half the LDS of a tile for
 // two more barriers per exchange.
-template <typename T, int L, int SV, int S, bool HX, typename LdsAt>
+//
+// ST > 0 is the distance between consecutive LDS indices of `lat` (1 for
+// row-mapped tiles, CT for column-mapped ones).  Where an exchange touches
+// index i0 + k*d with d a multiple of 16 (or i0 + r, r < R, with R and i0
+// multiples of 16), the accesses are then written as one address plus
+// constant offsets, which the LDS instructions take as immediates instead of
+// one address register per element.
+template <typename T, int L, int SV, int S, bool HX, int ST = 0, typename LdsAt>
 __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat, const cx<T>* __restrict__ tw) {
   constexpr int E = Sched<L, SV>::E;
   constexpr int TPT = L / E;
@@ -153,6 +167,15 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
   constexpr int NS = StageNs<L, SV, S>::value;
   constexpr int NB = E / R;
   constexpr bool LAST = (S == Sched<L, SV>::NST - 1);
+  // entry offset of write r (from base) and of read m (from t), or -1 where
+  // the padded index is not affine
+  auto woff = [](int r) constexpr -> int {
+    if constexpr (ST > 0 && pad_step<NS>() > 0) return r * pad_step<NS>() * ST;
+    if constexpr (ST > 0 && NS == 1 && R % 16 == 0) return (r + r / 16) * ST;
+    return -1;
+  };
+  constexpr bool WAFF = ST > 0 && (pad_step<NS>() > 0 || (NS == 1 && R % 16 == 0));
+  constexpr bool RAFF = ST > 0 && pad_step<TPT>() > 0;
   static_for<NB>([&](auto bc) AFGPU_INLINE {
     constexpr int b = decltype(bc)::value;
     cx<T> a[R];
@@ -170,46 +193,55 @@ __device__ __forceinline__ void run_stages(Regs<T, L, SV>& v, int t, LdsAt& lat,
     static_for<R>([&](auto rc) AFGPU_INLINE { v[b + decltype(rc)::value * NB] = a[decltype(rc)::value]; });
   });
   if constexpr (!LAST) {
+    // entry r of butterfly b, and entry t + m*TPT
+    auto wref = [&](auto bc, auto rc) AFGPU_INLINE -> decltype(lat(0))& {
+      constexpr int b = decltype(bc)::value;
+      constexpr int r = decltype(rc)::value;
+      const int j = t + b * TPT;
+      const int base = (j / NS) * (NS * R) + (j & (NS - 1));
+      if constexpr (WAFF)
+        return (&lat(base))[woff(r)];
+      else
+        return lat(base + r * NS);
+    };
+    auto* const rbase = &lat(t);
+    auto rref = [&](auto mc) AFGPU_INLINE -> decltype(lat(0))& {
+      constexpr int m = decltype(mc)::value;
+      if constexpr (RAFF)
+        return rbase[m * pad_step<TPT>() * ST];
+      else
+        return lat(t + m * TPT);
+    };
     if constexpr (!HX) {
       static_for<NB>([&](auto bc) AFGPU_INLINE {
         constexpr int b = decltype(bc)::value;
-        const int j = t + b * TPT;
-        const int base = (j / NS) * (NS * R) + (j & (NS - 1));
-        static_for<R>([&](auto rc) AFGPU_INLINE {
-          constexpr int r = decltype(rc)::value;
-          lat(base + r * NS) = v[b + r * NB];
-        });
+        static_for<R>([&](auto rc) AFGPU_INLINE { wref(bc, rc) = v[b + decltype(rc)::value * NB]; });
       });
       __syncthreads();
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        v[m] = lat(t + m * TPT);
-      });
+      static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = rref(mc); });
       __syncthreads();
     } else {
       static_for<2>([&](auto hc) AFGPU_INLINE {
         constexpr int h = decltype(hc)::value;
         static_for<NB>([&](auto bc) AFGPU_INLINE {
           constexpr int b = decltype(bc)::value;
-          const int j = t + b * TPT;
-          const int base = (j / NS) * (NS * R) + (j & (NS - 1));
           static_for<R>([&](auto rc) AFGPU_INLINE {
             constexpr int r = decltype(rc)::value;
-            lat(base + r * NS) = h ? v[b + r * NB].y : v[b + r * NB].x;
+            wref(bc, rc) = h ? v[b + r * NB].y : v[b + r * NB].x;
           });
         });
         __syncthreads();
         static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
           if constexpr (h)
-            v[m].y = lat(t + m * TPT);
+            v[m].y = rref(mc);
           else
-            v[m].x = lat(t + m * TPT);
+            v[m].x = rref(mc);
         });
         __syncthreads();
       });
     }
-    run_stages<T, L, SV, S + 1, HX>(v, t, lat, tw);
+    run_stages<T, L, SV, S + 1, HX, ST>(v, t, lat, tw);
   }
 }
 
@@ -437,9 +469,9 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   }
 
   if constexpr (HX)
-    run_stages<T, L, SV, 0, true>(v, t, latf, p.stage_tw);
+    run_stages<T, L, SV, 0, true, COLMAP ? CT : 1>(v, t, latf, p.stage_tw);
   else
-    run_stages<T, L, SV, 0, false>(v, t, lat, p.stage_tw);
+    run_stages<T, L, SV, 0, false, COLMAP ? CT : 1>(v, t, lat, p.stage_tw);
 
   // The output policy is hoisted over the whole epilogue for column tiles
   // (fewer live registers in their twiddled store); row tiles keep one branch

@@ -112,7 +112,7 @@ __global__ void __launch_bounds__((PassShape<L, 2 * R, SV>::NT), (rp_min_waves<T
     constexpr int m = decltype(mc)::value;
     v[m] = buf_ld<T, 0>(rin, rowoff + (unsigned)(m * TPT) * ES);
   });
-  run_stages<T, L, SV, 0, true>(v, t, latf, a.stage_tw);
+  run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
 
   // real post-pass: partner real parts, then imaginary parts, through the LDS
   const cx<T> wr = a.real_tw[row];
