@@ -65,6 +65,24 @@ struct RealPairArgs {
   const cx<T>* row_tw;   // W_{2 n2}^k, k < n2  (= W_{2h}^(n1 k))
 };
 
+// N-D r2c: the real pass over dst row pairs (k, h-k) fused into the FFT along
+// the last axis (L points, rows of dst in place; afft_realpair.hip).  Row r of
+// batch entry b starts at element b*bs + r*rs (complex units when il, scalar
+// otherwise); rows 0..h-1 hold Z on entry, rows 0..h hold the r2c on exit.
+template <typename T>
+struct RealRowFftArgs {
+  T* re;
+  T* im;
+  int il;
+  int L;
+  idx_t h, rs, bs, nbatch;
+  const cx<T>* stage_tw;  // stage twiddles of the L-point rows (radix-16 schedule)
+  const cx<T>* real_tw;   // W_{2h}^k, k < h
+};
+
+// all = false: only the row lengths where the fused pass is the default
+bool real_rowfft_supported(int elem_bytes, idx_t h, int L, bool all);
+
 // Whether real_pair has a kernel for rows of n2 points and n1 rows.
 bool real_pair_supported(int elem_bytes, int n1, int n2);
 
@@ -80,6 +98,8 @@ template <typename T>
 hipError_t real_tr(const RealTrArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_pair(const RealPairArgs<T>& a, hipStream_t st);
+template <typename T>
+hipError_t real_rowfft(const RealRowFftArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

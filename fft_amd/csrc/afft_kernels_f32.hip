@@ -18,4 +18,5 @@
 #define AFGPU_COL_CT2048 8
 #define AFGPU_COL_CT4096 4
 #define AFGPU_DUAL 1
+#define AFGPU_WIDE_COL1024 1
 #include "afft_variants.inc"

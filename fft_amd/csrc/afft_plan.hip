@@ -69,7 +69,7 @@ constexpr int kMaxDim = 64;
 constexpr int kMaxPass = 160;
 
 enum Role { R_SRC = 0, R_DST = 1, R_SCR = 2, R_SCR2 = 3 };
-enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3, PT_REALPAIR = 4 };
+enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3, PT_REALPAIR = 4, PT_REALROWFFT = 5 };
 
 struct Side {
   int role = R_SRC;
@@ -119,7 +119,7 @@ struct Variant {
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
   int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
-  int real_pair = 0;  // long 1-D r2c: 1 = pass B fused with the real post-pass where compiled, 0 (default) / -1 = not
+  int real_pair = 0;  // fused real passes: 1 = all (also 1-D r2c pass B), 0 (default) = the N-D row-pair one, -1 = none
 };
 
 struct Plan {
@@ -605,7 +605,7 @@ struct Builder {
   // h = h1 * h2 (pass A: h1-point columns of rows n1*h2 + n2, batch index
   // n2, twiddle W_h^(n2*k1); pass B: h2-point columns, output row k1 + h1*k2
   // of dst), then the real pass over row pairs (k, h-k) in place on dst.
-  void real_dim0_fourstep_r2c(const Side& usr_src, const Side& usr_dst, idx_t n0, idx_t inner) {
+  void real_dim0_fourstep_r2c(const Side& usr_src, const Side& usr_dst, idx_t n0, idx_t inner, bool real_pass = true) {
     const idx_t h = n0 / 2;
     const idx_t h1 = (idx_t)1 << (ilog2(h) / 2), h2 = h / h1;
     need_scratch(R_SCR, h * inner);
@@ -653,6 +653,7 @@ struct Builder {
     b->n_col = inner;
     b->n_blo = (int)h1;
     b->b_per_user = h1;
+    if (!real_pass) return;
     Side rr = usr_dst;
     rr.s_j = inner;
     rr.s_c = 1;
@@ -840,7 +841,30 @@ struct Builder {
         return ok;
       }
       if (h > kColDirectMax) {
-        real_dim0_fourstep_r2c(usr_src, usr_dst, n0, inner);
+        // 2-D with a compiled row length: the real pass over row pairs runs
+        // inside the row FFTs (afft_realpair.hip) instead of on its own
+        const bool fuse =
+            nd == 2 && P.var.real_pair >= 0 && real_rowfft_supported((int)sizeof(T), h, (int)inner, P.var.real_pair > 0);
+        real_dim0_fourstep_r2c(usr_src, usr_dst, n0, inner, !fuse);
+        if (fuse) {
+          PassRec* r = add();
+          if (!r) return false;
+          r->type = PT_REALROWFFT;
+          r->L = (int)inner;
+          r->CT = 2;
+          r->hx = 1;
+          r->in = usr_dst;
+          r->in.s_j = 1;
+          r->in.s_c = inner;
+          r->in.s_b = r->in.s_u = (h + 1) * inner;
+          r->out = r->in;
+          r->rr_h = h;
+          r->rr_m = inner;
+          r->stage_tw = get_table<T>(0, inner, 0);
+          r->real_tw = get_table<T>(2, h, 0);
+          if (!r->stage_tw || !r->real_tw) ok = false;
+          return ok;
+        }
         inner_axes_on_dst(usr_dst, d, nd, h);
         return ok;
       }
@@ -1120,6 +1144,21 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.nbatch = cnt;
     a.real_tw = (const cx<T>*)r.real_tw;
     HIP_OK(real_rows<T>(a, r.rr_inv != 0, st));
+    return true;
+  }
+  if (r.type == PT_REALROWFFT) {
+    RealRowFftArgs<T> a{};
+    a.re = ore;
+    a.im = oim;
+    a.il = r.out.il;
+    a.L = r.L;
+    a.h = r.rr_h;
+    a.rs = r.out.s_c;
+    a.bs = r.out.s_b;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(real_rowfft<T>(a, st));
     return true;
   }
   if (r.type == PT_REALPAIR) {
@@ -1474,11 +1513,11 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
               if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
               cands.push_back(v);
             }
-  // long 1-D r2c plans also try the other form of the real pass
+  // plans with a real pass that has a fused form also try the other form
   for (int i = 0; i < P.npass; i++)
-    if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR) {
+    if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT) {
       Variant v;
-      v.real_pair = P.pass[i].type == PT_REALPAIR ? -1 : 1;
+      v.real_pair = P.pass[i].type == PT_REALTR ? 1 : -1;
       cands.push_back(v);
       break;
     }
@@ -1792,6 +1831,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALTR)
     snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
+  else if (r.type == PT_REALROWFFT)
+    snprintf(buf, len, "real_rowfft L=%d h=%lld out(role=%d il=%d)", r.L, (long long)r.rr_h, r.out.role, r.out.il);
   else if (r.type == PT_REALPAIR)
     snprintf(buf, len, "real_pair L=%d rows=%d h=%lld n1=%lld out(role=%d il=%d)", r.L, r.CT, (long long)r.rr_h,
              (long long)r.rr_m, r.out.role, r.out.il);

@@ -1,5 +1,6 @@
-// afft_realpair.hip -- pass B of the four-step 1-D r2c with the real post-pass
-// and the output transposition fused in.
+// afft_realpair.hip -- real passes fused into FFT passes: pass B of the
+// four-step 1-D r2c with the real post-pass and the output transposition, and
+// the N-D r2c real pass over row pairs with the first inner-axis row pass.
 //
 // A long r2c (N reals, h = N/2 = n1 * n2 complex points) runs its h-point c2c
 // as a four-step: pass A (n1-point columns with the W_h twiddle) leaves
@@ -196,6 +197,86 @@ __global__ void __launch_bounds__((PassShape<L, 2 * R, SV>::NT), (rp_min_waves<T
     store_all(std::integral_constant<int, 0>{});
 }
 
+// ---- N-D r2c: real pass over row pairs fused into the first inner-axis pass --
+//
+// After the dim-0 c2c of an N-D r2c, dst rows 0..h-1 hold Z[k][j] (j = index
+// along the untransformed last axis).  The real pass (multi.hpp:343-414,
+// inverse=false) is elementwise per column j between rows k and h-k:
+//     X[k]   = (Z[k] + conj Z[h-k]) / 2 - i W_2h^k   (Z[k] - conj Z[h-k]) / 2,
+//     X[h-k] = the same with k and h-k swapped (W_2h^(h-k) = -conj W_2h^k),
+// X[0] = Re Z[0] + Im Z[0], X[h] = Re Z[0] - Im Z[0], X[h/2] = conj Z[h/2].
+// Work-group p (0 <= p <= h/2) reads rows p and h-p, forms X for both (slot 0:
+// row p, slot 1: row h-p; p = 0: rows 0 and h, both from Z[0]; p = h/2: slot 0
+// only), runs the L-point FFT along each row and writes the rows in place:
+// the real pass costs no pass of its own.  With the own and the other row's
+// value at the same j in hand, the uniform formula
+//     X = (own + conj oth) / 2 - i w (own - conj oth) / 2,
+//     w = W_2h^p (slot 0), -conj W_2h^p (slot 1)
+// covers every case (p = 0 gives Re +- Im, p = h/2 gives conj Z).
+// FP32: two work-groups per CU where their LDS fits (registers bounded to
+// match); FP64 rows need the registers of one.
+template <typename T, int L, int SV>
+constexpr int rrf_min_waves() {
+  constexpr int nt = PassShape<L, 2, SV>::NT;
+  constexpr int groups =
+      sizeof(T) == 4 && 2 * PassShape<L, 2, SV>::template lds_bytes<T, true>() <= 160 * 1024 ? 2 : 1;
+  return groups * nt / 256 > 1 ? groups * nt / 256 : 1;
+}
+
+template <typename T, int L, int SV>
+__global__ void __launch_bounds__((PassShape<L, 2, SV>::NT), (rrf_min_waves<T, L, SV>()))
+    real_rowfft_kernel(RealRowFftArgs<T> a) {
+  using PS = PassShape<L, 2, SV>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  constexpr int LL = PS::LL;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const idx_t np = a.h / 2 + 1;
+  const idx_t b = blockIdx.x / np;
+  const idx_t p = blockIdx.x - b * np;
+  const int tid = threadIdx.x;
+  const int c = tid / TPT, t = tid % TPT;
+  const idx_t h = a.h;
+  const idx_t own_row = c == 0 ? p : (p == 0 ? 0 : h - p);
+  const idx_t oth_row = c == 0 ? (p == 0 ? 0 : h - p) : p;
+  const idx_t out_row = c == 0 ? p : (p == 0 ? h : h - p);
+  const bool store = !(c == 1 && 2 * p == h);
+  const cx<T> wp = a.real_tw[p];
+  const cx<T> w = c == 0 ? wp : cx<T>{-wp.x, wp.y};
+  const idx_t base = b * a.bs;
+  auto ld = [&](idx_t row, int j) -> cx<T> {
+    const idx_t e = base + row * a.rs + j;
+    if (a.il) return reinterpret_cast<const cx<T>*>(a.re)[e];
+    return cx<T>{a.re[e], a.im[e]};
+  };
+  // eight elements (sixteen loads) at a time: hoisting all of them would
+  // double the registers the row needs
+  cx<T> v[E];
+  constexpr int G = E < 8 ? E : 8;
+  static_for<E / G>([&](auto gc) AFGPU_INLINE {
+    static_for<G>([&](auto ic) AFGPU_INLINE {
+      constexpr int m = G * decltype(gc)::value + decltype(ic)::value;
+      const int j = t + m * TPT;
+      const cx<T> own = ld(own_row, j), oth = ld(oth_row, j);
+      const T hf = T(0.5);
+      const cx<T> oc = conjg(oth);
+      const cx<T> s = {(own.x + oc.x) * hf, (own.y + oc.y) * hf};
+      const cx<T> dw = cmul(own - oc, w);
+      v[m] = s + mul_neg_i(cx<T>{dw.x * hf, dw.y * hf});
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
+  run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
+  if (store) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      st_elem(a.re, a.im, a.il, base + out_row * a.rs + t + m * TPT, v[m], 0);
+    });
+  }
+}
+
 namespace {
 constexpr int kRpR = 8;  // row pairs per work-group: 8 consecutive bins per store run
 
@@ -234,6 +315,46 @@ hipError_t real_pair(const RealPairArgs<T>& a, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
+template <typename T, int L>
+static hipError_t launch_rrf(const RealRowFftArgs<T>& a, idx_t blocks, hipStream_t st) {
+  constexpr int SV = 0;
+  constexpr size_t lds = PassShape<L, 2, SV>::template lds_bytes<T, true>();
+  auto k = real_rowfft_kernel<T, L, SV>;
+  if (lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(PassShape<L, 2, SV>::NT), lds, st, a);
+  return hipGetLastError();
+}
+
+bool real_rowfft_supported(int elem_bytes, idx_t h, int L, bool all) {
+  (void)elem_bytes;
+  // 1024..4096-point rows measured 0-11 % faster than the real pass plus the
+  // row pass; 8192-point rows (in a full 8192^2 r2c) 11 % slower: on request
+  return h >= 2 && (L == 1024 || L == 2048 || L == 4096 || (all && L == 8192));
+}
+
+template <typename T>
+hipError_t real_rowfft(const RealRowFftArgs<T>& a, hipStream_t st) {
+  if (!real_rowfft_supported((int)sizeof(T), a.h, a.L, true)) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (a.h / 2 + 1);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > INT_MAX) return hipErrorInvalidValue;
+  switch (a.L) {
+    case 1024: return launch_rrf<T, 1024>(a, blocks, st);
+    case 2048: return launch_rrf<T, 2048>(a, blocks, st);
+    case 4096: return launch_rrf<T, 4096>(a, blocks, st);
+    default: return launch_rrf<T, 8192>(a, blocks, st);
+  }
+}
+
+template hipError_t real_rowfft<float>(const RealRowFftArgs<float>&, hipStream_t);
+template hipError_t real_rowfft<double>(const RealRowFftArgs<double>&, hipStream_t);
 template hipError_t real_pair<float>(const RealPairArgs<float>&, hipStream_t);
 template hipError_t real_pair<double>(const RealPairArgs<double>&, hipStream_t);
 

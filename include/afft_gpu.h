@@ -104,9 +104,13 @@ typedef struct
   int32_t pipeline;  /* two-pass four-step plans over several groups: 1 runs pass A of
                         group k and pass B of group k-1 in one fused launch (double-
                         buffered scratch); 0 (default) or -1: one pass per launch */
-  int32_t real_pair; /* FP32 1-D r2c, 2^16 <= N <= 2^24: 1 runs pass B of the four-step
-                        with the real post-pass and the output transposition fused in
-                        (rows k1 and n1-k1 per work-group; half the scratch, measured
+  int32_t real_pair; /* fused real passes.  2-D r2c with dim0 > 4096 and a 1024..8192-point
+                        last axis: the real pass over row pairs (k, h-k) runs inside
+                        the row FFTs (0: rows up to 4096 points, 1: all, -1: a pass
+                        of its own).  FP32 1-D r2c,
+                        2^16 <= N <= 2^24: 1 runs pass B of the four-step with the
+                        real post-pass and the output transposition fused in (rows
+                        k1 and n1-k1 per work-group; half the scratch, measured
                         5-10 % slower); 0 (default) or -1: a row pass into a second
                         scratch, then the transposing real pass */
 } afft_gpu_variant;
@@ -196,7 +200,9 @@ typedef struct
   int32_t type; /* 0 = FFT pass, 1 = real-rows pass, 2 = single-value real,
                    3 = transposing real pass (rr_h = h, rr_m = n1),
                    4 = pass B of a 1-D r2c four-step fused with the real pass
-                       (L = n2-point rows of the input, rr_h = h, rr_m = n1) */
+                       (L = n2-point rows of the input, rr_h = h, rr_m = n1),
+                   5 = N-D r2c real pass over row pairs fused into the L-point
+                       row FFTs, in place (rr_h = h, row stride in.s_c) */
   int32_t L, CT, colmap, epi, store_t;
   afft_gpu_side_info in, out;
   int64_t n_col;

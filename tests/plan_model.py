@@ -309,6 +309,35 @@ def _real_pair(p, bufs, start, cnt, pi):
         raise AssertionError(f"pass {pi}: rows not covered exactly once")
 
 
+def _real_rowfft(p, bufs, start, cnt, pi):
+    """N-D r2c real pass over row pairs fused into the row FFTs
+    (afft_realpair.hip real_rowfft_kernel), in place: work-group q reads rows q
+    and h-q (q = 0: row 0 twice), forms X for its output rows (q, h-q; q = 0:
+    rows 0 and h; q = h/2: row h/2 once) and writes their L-point FFTs."""
+    h, L = p.rr_h, p.L
+    uoff = start * p.out.s_u if p.out.role < 2 else 0
+    q = np.arange(h // 2 + 1)
+    # (q, slot) pairs in work-group order; slot 1 of q = h/2 stores nothing
+    qq = np.concatenate([q, q[q * 2 != h]])
+    c = np.concatenate([np.zeros(q.size, dtype=np.int64), np.ones(qq.size - q.size, dtype=np.int64)])
+    own_row = np.where(c == 0, qq, np.where(qq == 0, 0, h - qq))
+    oth_row = np.where(c == 0, np.where(qq == 0, 0, h - qq), qq)
+    out_row = np.where(c == 0, qq, np.where(qq == 0, h, h - qq))
+    if np.bincount(out_row, minlength=h + 1).tolist() != [1] * (h + 1):
+        raise AssertionError(f"pass {pi}: rows not written exactly once")
+    wq = np.exp(-2j * np.pi * qq / (2 * h))
+    w = np.where(c == 0, wq, -np.conj(wq)).reshape(1, -1, 1)
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    j = np.arange(L).reshape(1, 1, L)
+    base = uoff + b * p.out.s_b
+    own = _gather(p.in_, bufs, base + own_row.reshape(1, -1, 1) * p.in_.s_c + j * p.in_.s_j, f"pass {pi} load")
+    oth = _gather(p.in_, bufs, base + oth_row.reshape(1, -1, 1) * p.in_.s_c + j * p.in_.s_j, f"pass {pi} load")
+    x = 0.5 * (own + np.conj(oth)) - 1j * 0.5 * w * (own - np.conj(oth))
+    y = np.fft.fft(x, axis=-1)
+    # every work-group reads its rows before writing them (rows are disjoint)
+    _scatter(p.out, bufs, base + out_row.reshape(1, -1, 1) * p.out.s_c + j * p.out.s_j, y, f"pass {pi} store")
+
+
 def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
     """Executes the plan model; returns the dst arrays (float64).  edit
     (optional) maps the inspected pass list to the one executed."""
@@ -332,6 +361,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
                 _real_tr(p, bufs, start, cnt, pi)
             elif p.type == 4:
                 _real_pair(p, bufs, start, cnt, pi)
+            elif p.type == 5:
+                _real_rowfft(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

@@ -153,6 +153,16 @@ def test_plan_model_matches_oracle(kind, dims):
             assert err < 1e-9, (kind, dims, fmt, batch, err)
 
 
+def test_plan_model_real_pass_in_rows():
+    # long real dim 0 with a 1024-point last axis: the real pass over row
+    # pairs runs inside the row FFTs (pass type 5)
+    for fmt in (IL, SP):
+        for batch in (1, 3):
+            err, info, passes = model_check("r", (8192, 1024), fmt, batch)
+            assert [p.type for p in passes] == [0, 0, 5]
+            assert err < 1e-9, (fmt, batch, err)
+
+
 @pytest.mark.parametrize("lg", [16, 17, 20, 22])
 def test_plan_model_fused_real_pass(lg):
     # the opt-in fused real pass (rows k1 and n1 - k1 per work-group, 1024- or
@@ -189,6 +199,12 @@ def test_plan_shapes_of_headline_configs():
     assert [(p.type, p.L, p.colmap, p.epi) for p in passes] == [(0, 64, 1, 1), (0, 64, 1, 0), (1, 1, 0, 0),
                                                                  (0, 8192, 0, 0)]
     assert passes[0].tw4_blo == 1 and passes[2].rr_h == 4096 and passes[2].rr_m == 8192
+    # 16384 x 2048 r2c: the real pass inside the 2048-point row FFTs
+    info, passes = M.inspect("r", (16384, 2048), IL, 1)
+    assert [(p.type, p.L) for p in passes][-1] == (5, 2048) and passes[-1].rr_h == 8192
+    # 8192 x 64 r2c (no fused row kernel for 64-point rows): the real pass of its own
+    info, passes = M.inspect("r", (8192, 64), IL, 1)
+    assert [p.type for p in passes] == [0, 0, 1, 0] and passes[2].rr_h == 4096 and passes[2].rr_m == 64
     # 2048 x 8192 r2c: dim 0 short enough for one fused column r2c pass
     info, passes = M.inspect("r", (2048, 8192), IL, 1)
     assert [(p.L, p.colmap, p.epi) for p in passes] == [(1024, 1, 2), (8192, 0, 0)]

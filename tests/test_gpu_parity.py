@@ -88,7 +88,9 @@ def test_real_1d(G, lg):
 ND_SHAPES = [(2, 4), (4, 1, 8), (16, 32), (32, 16), (8, 8, 8), (2, 2, 2, 2), (8192, 4), (4, 32768), (64, 64, 64),
              (2, 8192), (32, 16384),
              # long strided axes and long real dim 0 (column four-step, real pass over row pairs)
-             (16384, 8), (4096, 4, 4), (8192, 64)]
+             (16384, 8), (4096, 4, 4), (8192, 64),
+             # long real dim 0 with 1024..8192-point rows: the real pass inside the row FFTs
+             (8192, 1024), (8192, 2048)]
 
 
 @pytest.mark.parametrize("shape", ND_SHAPES, ids=lambda s: "x".join(map(str, s)))
@@ -206,8 +208,18 @@ def test_large_nd_against_afft_summaries(G):
             m = int(np.prod(G.half(shape)))
         dst = G.empty_dev(2 * m)
         dst64 = G.empty_dev(2 * m, np.float64)
-        Plan(kind, shape)(src, None, dst, None)
         Plan(kind, shape, dtype=np.float64)(src.double(), None, dst64, None)
+        if kind == "r":
+            # the opt-in fused real passes (real pass inside the 8192-point
+            # rows; 1-D pass B with the real post-pass) at full size
+            pf = Plan(kind, shape)
+            pf.set_variant(real_pair=1)
+            pf(src, None, dst, None)
+            torch.cuda.synchronize()
+            say("  fused", pf.describe()[-1])
+            assert G.device_rel_l2(dst, dst64) <= O.bound(n)
+            pf.close()
+        Plan(kind, shape)(src, None, dst, None)
         torch.cuda.synchronize()
         assert G.device_rel_l2(dst, dst64) <= O.bound(n)
         y = G.to_host(dst)
