@@ -80,6 +80,22 @@ struct RealRowFftArgs {
   const cx<T>* real_tw;   // W_{2h}^k, k < h
 };
 
+// N-D c2r: the inverse real pass over row pairs fused into the inverse FFT
+// along the last axis: input rows 0..h of the user's spectrum (in_rs / in_bs,
+// complex units when in_il, scalar otherwise), output rows 0..h-1 of Z into
+// the plan's scratch (interleaved; afft_realpair.hip).
+template <typename T>
+struct RealIRowFftArgs {
+  const T* in_re;
+  const T* in_im;
+  int in_il;
+  cx<T>* out;
+  int L;
+  idx_t h, in_rs, in_bs, out_rs, out_bs, nbatch;
+  const cx<T>* stage_tw;  // stage twiddles of the L-point rows (radix-16 schedule)
+  const cx<T>* real_tw;   // W_{2h}^k, k < h
+};
+
 // all = false: only the row lengths where the fused pass is the default
 bool real_rowfft_supported(int elem_bytes, idx_t h, int L, bool all);
 
@@ -100,6 +116,8 @@ template <typename T>
 hipError_t real_pair(const RealPairArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_rowfft(const RealRowFftArgs<T>& a, hipStream_t st);
+template <typename T>
+hipError_t real_irowfft(const RealIRowFftArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

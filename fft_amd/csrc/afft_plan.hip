@@ -69,7 +69,15 @@ constexpr int kMaxDim = 64;
 constexpr int kMaxPass = 160;
 
 enum Role { R_SRC = 0, R_DST = 1, R_SCR = 2, R_SCR2 = 3 };
-enum PassType { PT_FFT = 0, PT_REALROWS = 1, PT_REALONE = 2, PT_REALTR = 3, PT_REALPAIR = 4, PT_REALROWFFT = 5 };
+enum PassType {
+  PT_FFT = 0,
+  PT_REALROWS = 1,
+  PT_REALONE = 2,
+  PT_REALTR = 3,
+  PT_REALPAIR = 4,
+  PT_REALROWFFT = 5,
+  PT_REALIROWFFT = 6
+};
 
 struct Side {
   int role = R_SRC;
@@ -271,7 +279,10 @@ struct Builder {
     // column passes without an explicit tile first try a half-LDS kernel with
     // twice the default tile (wider row segments at the same LDS footprint);
     // short columns (<= 128 points) a 4x-wide tile, whose LDS is still small
-    const bool wide = want <= 0 && colmap && P.var.half_lds >= 0;
+    // (not for 1024-point columns: 32-column tiles, one group per CU, measured
+    // 0.285 against 0.255 ms per 64 x 2^20 direction for two 16-column groups;
+    // they stay a variant, col_tile = 32)
+    const bool wide = want <= 0 && colmap && P.var.half_lds >= 0 && L != 1024;
     if (pref_ct <= 0 && want <= 0 && colmap && L <= 128) pref_ct = 4 * deflt;
     struct Cand {
       int ct, hx_only;
@@ -665,14 +676,16 @@ struct Builder {
   // after the inner axes (plan scratch); the inverse real pass runs in place
   // on it, then the inverse h-point c2c as a column four-step whose pass B
   // writes row k1 + h1*k2 as the real rows 2(k1 + h1*k2), 2(k1 + h1*k2) + 1.
-  void real_dim0_fourstep_c2r(const Side& cur, const Side& usr_dst, idx_t n0, idx_t inner) {
+  void real_dim0_fourstep_c2r(const Side& cur, const Side& usr_dst, idx_t n0, idx_t inner, bool real_pass = true) {
     const idx_t h = n0 / 2;
     const idx_t h1 = (idx_t)1 << (ilog2(h) / 2), h2 = h / h1;
-    Side rr = cur;
-    rr.s_j = inner;
-    rr.s_c = 1;
-    rr.s_b = (h + 1) * inner;
-    real_rows(rr, rr, h, inner, 1);
+    if (real_pass) {
+      Side rr = cur;
+      rr.s_j = inner;
+      rr.s_c = 1;
+      rr.s_b = (h + 1) * inner;
+      real_rows(rr, rr, h, inner, 1);
+    }
     need_scratch(R_SCR2, h * inner);
     const int bits = (ilog2(h) + 1) / 2;
     const void* tw4 = get_table<T>(1, h, bits);
@@ -935,6 +948,33 @@ struct Builder {
     scratch_side(scr, R_SCR);
     need_scratch(R_SCR, (h + 1) * inner);
     Side cur = in;
+    // 2-D with a long dim 0 and a compiled row length: the inverse real pass
+    // over row pairs runs inside the row FFTs (afft_realpair.hip)
+    if (h > kColDirectMax && nd == 2 && P.var.real_pair >= 0 &&
+        real_rowfft_supported((int)sizeof(T), h, (int)inner, P.var.real_pair > 0)) {
+      PassRec* r = add();
+      if (!r) return false;
+      r->type = PT_REALIROWFFT;
+      r->L = (int)inner;
+      r->CT = 2;
+      r->hx = 1;
+      r->in = in;
+      r->in.s_j = 1;
+      r->in.s_c = inner;
+      r->in.s_b = in.s_u;
+      r->out = scr;
+      r->out.s_j = 1;
+      r->out.s_c = inner;
+      r->out.s_b = (h + 1) * inner;
+      r->rr_h = h;
+      r->rr_m = inner;
+      r->rr_inv = 1;
+      r->stage_tw = get_table<T>(0, inner, 0);
+      r->real_tw = get_table<T>(2, h, 0);
+      if (!r->stage_tw || !r->real_tw) ok = false;
+      real_dim0_fourstep_c2r(scr, usr_dst, n0, inner, false);
+      return ok;
+    }
     for (int a = nd - 1; a >= 1; a--) {
       idx_t outer = 1, inn = 1;
       for (int i = 0; i < a; i++) outer *= dd[i];
@@ -1144,6 +1184,24 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.nbatch = cnt;
     a.real_tw = (const cx<T>*)r.real_tw;
     HIP_OK(real_rows<T>(a, r.rr_inv != 0, st));
+    return true;
+  }
+  if (r.type == PT_REALIROWFFT) {
+    RealIRowFftArgs<T> a{};
+    a.in_re = ire;
+    a.in_im = iim;
+    a.in_il = r.in.il;
+    a.out = reinterpret_cast<cx<T>*>(ore);
+    a.L = r.L;
+    a.h = r.rr_h;
+    a.in_rs = r.in.s_c;
+    a.in_bs = r.in.s_b;
+    a.out_rs = r.out.s_c;
+    a.out_bs = r.out.s_b;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(real_irowfft<T>(a, st));
     return true;
   }
   if (r.type == PT_REALROWFFT) {
@@ -1497,7 +1555,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   }
   // the XCD order and streaming I/O stay manual knobs: neither has lost
   // (streaming) or won (XCD order) on a measured shape
-  for (int col : {0, 8})
+  for (int col : {0, 8, 32})
     for (int rowt : {0, 4, 16})
       for (int pl : {0, 1})
         for (int hl : {0, -1})
@@ -1515,7 +1573,8 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
             }
   // plans with a real pass that has a fused form also try the other form
   for (int i = 0; i < P.npass; i++)
-    if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT) {
+    if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT ||
+        P.pass[i].type == PT_REALIROWFFT) {
       Variant v;
       v.real_pair = P.pass[i].type == PT_REALTR ? 1 : -1;
       cands.push_back(v);
@@ -1831,6 +1890,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALTR)
     snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
+  else if (r.type == PT_REALIROWFFT)
+    snprintf(buf, len, "real_irowfft L=%d h=%lld in(role=%d il=%d)", r.L, (long long)r.rr_h, r.in.role, r.in.il);
   else if (r.type == PT_REALROWFFT)
     snprintf(buf, len, "real_rowfft L=%d h=%lld out(role=%d il=%d)", r.L, (long long)r.rr_h, r.out.role, r.out.il);
   else if (r.type == PT_REALPAIR)

@@ -277,6 +277,86 @@ __global__ void __launch_bounds__((PassShape<L, 2, SV>::NT), (rrf_min_waves<T, L
   }
 }
 
+// ---- N-D c2r: inverse real pass over row pairs fused into the row FFTs -------
+//
+// A c2r first runs the inverse FFT along the last axis of the h+1 input rows
+// (Y = IDFT X), then the inverse real pass (multi.hpp:343-414, inverse=true)
+// pairs rows k and h-k elementwise:
+//     Z[k]   = (Y[k] + conj Y[h-k]) - i conj(W_2h^k) (conj Y[h-k] - Y[k]),
+//     Z[h-k] = its conjugate-symmetric partner,  Z[0] = (Re Y[0] + Re Y[h])
+//              + i (Re Y[0] - Re Y[h]).
+// Both steps are linear, and conj Y[r] is the IDFT of conj X[r][(-m) mod L],
+// so the pairing can run before the row transform on the input rows with the
+// partner row read at the mirrored column:
+//     pre[m] = (own[m] + conj oth[-m]) + i u (own[m] - conj oth[-m]),
+//     u = conj W_2h^p (slot 0, row p), -W_2h^p (slot 1, row h-p),
+// and for row 0 the Hermitian parts H_r[m] = (X[r][m] + conj X[r][-m]) / 2 of
+// rows 0 and h: pre = (H_0 + H_h) + i (H_0 - H_h) (this is what taking the
+// real parts after the row transform amounts to).  Work-group p reads input
+// rows p and h-p (p = 0: rows 0 and h), writes rows p and h-p of Z (p = 0,
+// p = h/2: one row) into the plan's scratch; the real pass costs no pass.
+template <typename T, int L, int SV>
+__global__ void __launch_bounds__((PassShape<L, 2, SV>::NT), (rrf_min_waves<T, L, SV>()))
+    real_irowfft_kernel(RealIRowFftArgs<T> a) {
+  using PS = PassShape<L, 2, SV>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  constexpr int LL = PS::LL;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const idx_t np = a.h / 2 + 1;
+  const idx_t b = blockIdx.x / np;
+  const idx_t p = blockIdx.x - b * np;
+  const int tid = threadIdx.x;
+  const int c = tid / TPT, t = tid % TPT;
+  const idx_t h = a.h;
+  const bool row0 = p == 0 && c == 0;
+  const bool act = c == 0 || (p != 0 && 2 * p != h);
+  const idx_t own_row = c == 0 ? p : h - p;
+  const idx_t oth_row = c == 0 ? h - p : p;
+  const cx<T> wp = a.real_tw[p];
+  const cx<T> u = c == 0 ? conjg(wp) : cx<T>{-wp.x, -wp.y};
+  const idx_t ib = b * a.in_bs;
+  auto ld = [&](idx_t row, int j) -> cx<T> {
+    const idx_t e = ib + row * a.in_rs + j;
+    if (a.in_il) return reinterpret_cast<const cx<T>*>(a.in_re)[e];
+    return cx<T>{a.in_re[e], a.in_im[e]};
+  };
+  cx<T> v[E];
+  constexpr int G = E < 8 ? E : 8;
+  static_for<E / G>([&](auto gc) AFGPU_INLINE {
+    static_for<G>([&](auto ic) AFGPU_INLINE {
+      constexpr int m = G * decltype(gc)::value + decltype(ic)::value;
+      const int j = t + m * TPT;
+      const int jm = (L - j) & (L - 1);
+      cx<T> z;
+      if (row0) {
+        const T hf = T(0.5);
+        const cx<T> x0 = ld(0, j), x0m = conjg(ld(0, jm)), xh = ld(h, j), xhm = conjg(ld(h, jm));
+        const cx<T> h0 = {(x0.x + x0m.x) * hf, (x0.y + x0m.y) * hf};
+        const cx<T> hh = {(xh.x + xhm.x) * hf, (xh.y + xhm.y) * hf};
+        const cx<T> d = h0 - hh;
+        z = h0 + hh + cx<T>{-d.y, d.x};
+      } else {
+        const cx<T> own = ld(own_row, j), oc = conjg(ld(oth_row, jm));
+        const cx<T> du = cmul(own - oc, u);
+        z = own + oc + cx<T>{-du.y, du.x};
+      }
+      v[m] = swapri(z);  // inverse transform: swap, forward FFT, swap
+    });
+    __builtin_amdgcn_sched_barrier(0);
+  });
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
+  run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
+  if (act) {
+    cx<T>* __restrict__ out = a.out + b * a.out_bs + own_row * a.out_rs;
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      out[t + m * TPT] = swapri(v[m]);
+    });
+  }
+}
+
 namespace {
 constexpr int kRpR = 8;  // row pairs per work-group: 8 consecutive bins per store run
 
@@ -353,6 +433,39 @@ hipError_t real_rowfft(const RealRowFftArgs<T>& a, hipStream_t st) {
   }
 }
 
+template <typename T, int L>
+static hipError_t launch_irrf(const RealIRowFftArgs<T>& a, idx_t blocks, hipStream_t st) {
+  constexpr int SV = 0;
+  constexpr size_t lds = PassShape<L, 2, SV>::template lds_bytes<T, true>();
+  auto k = real_irowfft_kernel<T, L, SV>;
+  if (lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(PassShape<L, 2, SV>::NT), lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t real_irowfft(const RealIRowFftArgs<T>& a, hipStream_t st) {
+  if (!real_rowfft_supported((int)sizeof(T), a.h, a.L, true)) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (a.h / 2 + 1);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > INT_MAX) return hipErrorInvalidValue;
+  switch (a.L) {
+    case 1024: return launch_irrf<T, 1024>(a, blocks, st);
+    case 2048: return launch_irrf<T, 2048>(a, blocks, st);
+    case 4096: return launch_irrf<T, 4096>(a, blocks, st);
+    default: return launch_irrf<T, 8192>(a, blocks, st);
+  }
+}
+
+template hipError_t real_irowfft<float>(const RealIRowFftArgs<float>&, hipStream_t);
+template hipError_t real_irowfft<double>(const RealIRowFftArgs<double>&, hipStream_t);
 template hipError_t real_rowfft<float>(const RealRowFftArgs<float>&, hipStream_t);
 template hipError_t real_rowfft<double>(const RealRowFftArgs<double>&, hipStream_t);
 template hipError_t real_pair<float>(const RealPairArgs<float>&, hipStream_t);

@@ -92,7 +92,8 @@ int afft_gpu_mfma_selftest(void);
  * they differ in speed.  0 means "default" for the integer fields. */
 typedef struct
 {
-  int32_t col_tile;  /* columns per work-group of column passes with L >= 512 (8 or 16) */
+  int32_t col_tile;  /* columns per work-group of column passes with L >= 512 (8, 16; 32 for
+                        the 1024-point four-step columns of FP32 plans) */
   int32_t rowt_tile; /* rows per work-group of transposed-store row passes (4, 8 or 16) */
   int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
@@ -104,10 +105,10 @@ typedef struct
   int32_t pipeline;  /* two-pass four-step plans over several groups: 1 runs pass A of
                         group k and pass B of group k-1 in one fused launch (double-
                         buffered scratch); 0 (default) or -1: one pass per launch */
-  int32_t real_pair; /* fused real passes.  2-D r2c with dim0 > 4096 and a 1024..8192-point
-                        last axis: the real pass over row pairs (k, h-k) runs inside
-                        the row FFTs (0: rows up to 4096 points, 1: all, -1: a pass
-                        of its own).  FP32 1-D r2c,
+  int32_t real_pair; /* fused real passes.  2-D r2c / c2r with dim0 > 4096 and a
+                        1024..8192-point last axis: the real pass over row pairs
+                        (k, h-k) runs inside the row FFTs (0: rows up to 4096
+                        points, 1: all, -1: a pass of its own).  FP32 1-D r2c,
                         2^16 <= N <= 2^24: 1 runs pass B of the four-step with the
                         real post-pass and the output transposition fused in (rows
                         k1 and n1-k1 per work-group; half the scratch, measured
@@ -202,7 +203,10 @@ typedef struct
                    4 = pass B of a 1-D r2c four-step fused with the real pass
                        (L = n2-point rows of the input, rr_h = h, rr_m = n1),
                    5 = N-D r2c real pass over row pairs fused into the L-point
-                       row FFTs, in place (rr_h = h, row stride in.s_c) */
+                       row FFTs, in place (rr_h = h, row stride in.s_c),
+                   6 = N-D c2r inverse real pass over row pairs fused into the
+                       L-point inverse row FFTs (input rows 0..h, output rows
+                       0..h-1; rr_h = h, row strides in.s_c / out.s_c) */
   int32_t L, CT, colmap, epi, store_t;
   afft_gpu_side_info in, out;
   int64_t n_col;

@@ -338,6 +338,43 @@ def _real_rowfft(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, base + out_row.reshape(1, -1, 1) * p.out.s_c + j * p.out.s_j, y, f"pass {pi} store")
 
 
+def _real_irowfft(p, bufs, start, cnt, pi):
+    """N-D c2r inverse real pass over row pairs fused into the inverse row
+    FFTs (afft_realpair.hip real_irowfft_kernel): work-group q reads input rows
+    q and h-q (the partner at the mirrored column), writes rows q and h-q of Z
+    (q = 0: row 0 from the Hermitian parts of rows 0 and h; q = h/2: one row)."""
+    h, L = p.rr_h, p.L
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    q = np.arange(1, h // 2 + 1)
+    qq = np.concatenate([q, q[q * 2 != h]])
+    c = np.concatenate([np.zeros(q.size, dtype=np.int64), np.ones(qq.size - q.size, dtype=np.int64)])
+    own_row = np.where(c == 0, qq, h - qq)
+    oth_row = np.where(c == 0, h - qq, qq)
+    rows = np.concatenate([[0], own_row])
+    if np.bincount(rows, minlength=h).tolist() != [1] * h:
+        raise AssertionError(f"pass {pi}: rows not written exactly once")
+    wq = np.exp(-2j * np.pi * qq / (2 * h))
+    u = np.where(c == 0, np.conj(wq), -wq).reshape(1, -1, 1)
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    j = np.arange(L).reshape(1, 1, L)
+    jm = (L - j) % L
+    ib = uin + b * p.in_.s_b
+    own = _gather(p.in_, bufs, ib + own_row.reshape(1, -1, 1) * p.in_.s_c + j * p.in_.s_j, f"pass {pi} load")
+    oc = np.conj(_gather(p.in_, bufs, ib + oth_row.reshape(1, -1, 1) * p.in_.s_c + jm * p.in_.s_j,
+                         f"pass {pi} load"))
+    pre = (own + oc) + 1j * u * (own - oc)
+    x0 = _gather(p.in_, bufs, ib + 0 * p.in_.s_c + j * p.in_.s_j, f"pass {pi} load")
+    x0m = _gather(p.in_, bufs, ib + 0 * p.in_.s_c + jm * p.in_.s_j, f"pass {pi} load")
+    xh = _gather(p.in_, bufs, ib + h * p.in_.s_c + j * p.in_.s_j, f"pass {pi} load")
+    xhm = _gather(p.in_, bufs, ib + h * p.in_.s_c + jm * p.in_.s_j, f"pass {pi} load")
+    h0, hh = 0.5 * (x0 + np.conj(x0m)), 0.5 * (xh + np.conj(xhm))
+    pre0 = (h0 + hh) + 1j * (h0 - hh)
+    y = np.fft.ifft(np.concatenate([pre0, pre], axis=1), axis=-1) * L  # unnormalised inverse
+    ob = uout + b * p.out.s_b
+    _scatter(p.out, bufs, ob + rows.reshape(1, -1, 1) * p.out.s_c + j * p.out.s_j, y, f"pass {pi} store")
+
+
 def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
     """Executes the plan model; returns the dst arrays (float64).  edit
     (optional) maps the inspected pass list to the one executed."""
@@ -363,6 +400,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
                 _real_pair(p, bufs, start, cnt, pi)
             elif p.type == 5:
                 _real_rowfft(p, bufs, start, cnt, pi)
+            elif p.type == 6:
+                _real_irowfft(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

@@ -153,14 +153,15 @@ def test_plan_model_matches_oracle(kind, dims):
             assert err < 1e-9, (kind, dims, fmt, batch, err)
 
 
-def test_plan_model_real_pass_in_rows():
+@pytest.mark.parametrize("kind", ["r", "ri"])
+def test_plan_model_real_pass_in_rows(kind):
     # long real dim 0 with a 1024-point last axis: the real pass over row
-    # pairs runs inside the row FFTs (pass type 5)
+    # pairs runs inside the row FFTs (pass type 5, or 6 for c2r)
     for fmt in (IL, SP):
         for batch in (1, 3):
-            err, info, passes = model_check("r", (8192, 1024), fmt, batch)
-            assert [p.type for p in passes] == [0, 0, 5]
-            assert err < 1e-9, (fmt, batch, err)
+            err, info, passes = model_check(kind, (8192, 1024), fmt, batch)
+            assert [p.type for p in passes] == ([0, 0, 5] if kind == "r" else [6, 0, 0])
+            assert err < 1e-9, (kind, fmt, batch, err)
 
 
 @pytest.mark.parametrize("lg", [16, 17, 20, 22])

@@ -241,7 +241,7 @@ def test_errors_leave_state_clean(G):
 
 
 VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chunk=ch, real_pair=rp)
-            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8) for pl, ch in ((1, 1), (0, 0))
+            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8, 32) for pl, ch in ((1, 1), (0, 0))
             for rp in (1, -1)]
 
 
