@@ -30,7 +30,7 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_set_stream", "afft_gpu_plan_set_chunk", "afft_gpu_plan_num_passes",
     "afft_gpu_plan_scratch_bytes", "afft_gpu_plan_describe", "afft_gpu_last_error",
     "afft_gpu_error_sums_f32", "afft_gpu_error_sums_f64", "afft_gpu_copy", "afft_gpu_execute_range",
-    "afft_gpu_plan_launches", "afft_gpu_plan_inspect",
+    "afft_gpu_plan_launches", "afft_gpu_plan_inspect", "afft_gpu_plan_inspect_variant",
     "afft_gpu_queue_memsize", "afft_gpu_queue_create", "afft_gpu_queue_prepare", "afft_gpu_queue_launch",
     "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy", "afft_gpu_queue_status",
     "afft_gpu_queue_set_spin_limit", "afft_gpu_queue_set_groups_per_cu",

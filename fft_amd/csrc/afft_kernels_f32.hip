@@ -19,4 +19,5 @@
 #define AFGPU_COL_CT4096 4
 #define AFGPU_DUAL 1
 #define AFGPU_WIDE_COL1024 1
+#define AFGPU_C2R4 1
 #include "afft_variants.inc"

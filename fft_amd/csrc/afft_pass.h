@@ -35,6 +35,11 @@
 //              multi.hpp:343-414, inverse=false.
 //   EPI_C2R    inverse real pre-pass fused into the load (reads bins k and L-k,
 //              L+1 input bins): the inverse=true branch of the same functions.
+//   EPI_C2R4   pass A of the four-step inverse c2c of a long 1-D c2r: the
+//              inverse real pre-pass on the load, pairing input bin k (the
+//              element's offset in its batch entry, col*in_cs + j*in_js) with
+//              bin real_h - k, which lies in another column; then the EPI_TW4
+//              twiddle on the store (onedim.hpp:1031-1108, inverse=true).
 // STORE_T routes the final store through LDS so a row-mapped pass can write
 // with columns fastest (pass B of a four-step transform writes bins k2*N1+k1).
 #pragma once
@@ -43,7 +48,7 @@
 
 namespace afgpu {
 
-enum { EPI_PLAIN = 0, EPI_TW4 = 1, EPI_R2C = 2, EPI_C2R = 3 };
+enum { EPI_PLAIN = 0, EPI_TW4 = 1, EPI_R2C = 2, EPI_C2R = 3, EPI_C2R4 = 4 };
 
 template <typename T>
 struct PassParams {
@@ -68,7 +73,8 @@ struct PassParams {
   int tw4_lo_bits;
   int tw4_col_shift;      // n2 = column >> shift
   int tw4_blo;            // 1: n2 = low batch index instead (columns of one n2 are the c index)
-  const cx<T>* real_tw;   // W_{2L}^k, k < L
+  const cx<T>* real_tw;   // W_{2L}^k, k < L (EPI_C2R4: W_{2h}^k, k < h)
+  idx_t real_h;           // EPI_C2R4: h, the number of complex points
   // Byte extents of the in / out regions of one batch entry (fast path only).
   unsigned in_bytes, out_bytes;
   // Optional per-column buffers (uneven batches): column c reads from
@@ -335,6 +341,7 @@ struct RtPol {
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
   static_assert(!(HX && EPI == EPI_R2C), "the real-output epilogue exchanges whole complex entries");
+  static_assert(EPI != EPI_C2R4 || (COLMAP && !STORE_T), "the paired load is for four-step column passes");
   using PS = PassShape<L, CT, SV, (!COLMAP && STORE_T) ? (HX ? (int)sizeof(T) : 2 * (int)sizeof(T)) : 0>;
   constexpr int E = PS::E;
   constexpr int TPT = PS::TPT;
@@ -405,6 +412,14 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
       return ld_elem(in_re, in_im, p.in_il, ib + (idx_t)j * p.in_js, p.in_nt);
     }
   };
+  // element at offset e (elements) from the batch entry's base (EPI_C2R4)
+  auto load_at = [&](auto polc, idx_t e) AFGPU_INLINE -> cx<T> {
+    if constexpr (FAST) {
+      return buf_ld<T, decltype(polc)::value>(rin, (unsigned)e * ES);
+    } else {
+      return ld_elem(p.in_re, p.in_im, p.in_il, (idx_t)bhi * p.in_bs + (idx_t)blo * p.in_bs2 + e, p.in_nt);
+    }
+  };
   auto store = [&](auto polc, int k, cx<T> z) AFGPU_INLINE {
     if (p.swap_out) z = swapri(z);
     if constexpr (FAST) {
@@ -438,7 +453,37 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   cx<T> v[E];
   if (active) {
     with_pol(p.in_nt, [&](auto pol) AFGPU_INLINE {
-      if constexpr (EPI == EPI_C2R) {
+      if constexpr (EPI == EPI_C2R4) {
+        // Z[k] = (X[k] + conj X[h-k]) - i conj(W_2h^k) (conj X[h-k] - X[k]),
+        // Z[0] = (Re X[0] + Re X[h]) + i (Re X[0] - Re X[h]); eight elements at
+        // a time (two loads and a twiddle each)
+        const idx_t h = p.real_h;
+        const idx_t kc = (idx_t)col * p.in_cs;
+        // W_2h^k = W_2h^(col*in_cs) * W_2h^(j*in_js): one factor per thread and
+        // one per row of the tile, both from small, cached parts of the table
+        // (reading W_2h^k itself would stream a table as large as the data)
+        const cx<T> wc = p.real_tw[kc];
+        constexpr int G = E < 8 ? E : 8;
+        static_for<E / G>([&](auto gc) AFGPU_INLINE {
+          static_for<G>([&](auto ic) AFGPU_INLINE {
+            constexpr int m = G * decltype(gc)::value + decltype(ic)::value;
+            const int j = t + m * TPT;
+            const idx_t k = kc + (idx_t)j * p.in_js;
+            const cx<T> s0 = load(pol, j);
+            const cx<T> s1 = load_at(pol, h - k);
+            cx<T> z;
+            if (k == 0) {
+              z = {s0.x + s1.x, s0.x - s1.x};
+            } else {
+              const cx<T> s1c = conjg(s1);
+              const cx<T> bb = cmulc(s1c - s0, cmul(wc, p.real_tw[(idx_t)j * p.in_js]));
+              z = s0 + s1c + mul_neg_i(bb);
+            }
+            v[m] = p.swap_in ? swapri(z) : z;
+          });
+          __builtin_amdgcn_sched_barrier(0);
+        });
+      } else if constexpr (EPI == EPI_C2R) {
         static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
           const int k = t + m * TPT;
@@ -512,7 +557,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
         });
       }
     } else {
-      if constexpr (EPI == EPI_TW4 && !STORE_T) {
+      if constexpr ((EPI == EPI_TW4 || EPI == EPI_C2R4) && !STORE_T) {
         // twiddle and store eight bins at a time: the scheduling barrier keeps
         // the table loads of later groups from being hoisted, which would hold
         // every twiddle in registers at once.  Group g takes registers

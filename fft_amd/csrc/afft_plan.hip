@@ -931,6 +931,34 @@ struct Builder {
         r->swap_in = r->swap_out = 1;
         return ok;
       }
+      // long FP32 c2r whose four-step pass A has a kernel with the inverse
+      // real pre-pass fused into its load: pass A reads the spectrum, pairs
+      // bins k and h-k itself and no real pass runs on its own
+      if (P.var.real_pair > 0 && sizeof(T) == 4) {
+        const int np0 = P.npass;
+        axis(in, out, 1, h, 1, 1, 1);
+        PassRec* a = ok && P.npass == np0 + 2 ? &P.pass[np0] : nullptr;
+        if (a && a->type == PT_FFT && a->colmap && a->epi == EPI_TW4 && a->in.role == R_SRC) {
+          // the same kernel selection with the fused epilogue; a shape or
+          // variant without such a kernel keeps the unfused form
+          PassRec f = *a;
+          if (fft_pass(&f, a->L, 1, EPI_C2R4, 0)) {
+            a->epi = EPI_C2R4;
+            a->CT = f.CT;
+            a->variant = f.variant;
+            a->sv = f.sv;
+            a->hx = f.hx;
+            a->stage_tw = f.stage_tw;
+            a->rr_h = h;
+            a->real_tw = get_table<T>(2, h, 0);
+            if (!a->real_tw) ok = false;
+            return ok;
+          }
+        }
+        P.npass = np0;
+        ok = true;
+        g_err.clear();
+      }
       Side rin = in, rout = out;
       rin.s_j = rout.s_j = 1;
       rin.s_b = h + 1;
@@ -1101,7 +1129,12 @@ bool fft_params(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, Pass
   // path needs it below 4 GiB
   const idx_t lin = (idx_t)r.L + (r.epi == EPI_C2R ? 1 : 0);
   const idx_t lout = (idx_t)r.L + (r.epi == EPI_R2C ? 1 : 0);
-  const idx_t ext_in = ((ncol - 1) * r.in.s_c + (lin - 1) * r.in.s_j + 1) * (idx_t)sizeof(cx<T>);
+  idx_t ext_in = ((ncol - 1) * r.in.s_c + (lin - 1) * r.in.s_j + 1) * (idx_t)sizeof(cx<T>);
+  if (r.epi == EPI_C2R4) {
+    // the paired load reaches bin h of each batch entry
+    p.real_h = r.rr_h;
+    ext_in = std::max(ext_in, (r.rr_h + 1) * (idx_t)sizeof(cx<T>));
+  }
   const idx_t ext_out = ((ncol - 1) * r.out.s_c + (lout - 1) * r.out.s_j + 1) * (idx_t)sizeof(cx<T>);
   const bool fast_ok = r.in.il && r.out.il && ext_in > 0 && ext_out > 0 && ext_in < ((idx_t)1 << 32) &&
                        ext_out < ((idx_t)1 << 32);
@@ -1879,7 +1912,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   const Plan& P = *h.plan;
   if (i < 0 || i >= P.npass || !buf) return -1;
   const PassRec& r = P.pass[i];
-  static const char* ep[] = {"plain", "tw4", "r2c", "c2r"};
+  static const char* ep[] = {"plain", "tw4", "r2c", "c2r", "c2r+tw4"};
+  static_assert(sizeof(ep) / sizeof(ep[0]) == EPI_C2R4 + 1, "one name per epilogue");
   if (r.type == PT_FFT)
     snprintf(buf, len,
              "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
@@ -1921,6 +1955,12 @@ static void fill_side(const Side& s, afft_gpu_side_info& o) {
 
 int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
                           afft_gpu_plan_info* info, afft_gpu_pass_info* passes, int max_passes) {
+  return afft_gpu_plan_inspect_variant(precision, kind, ndim, dim, format, batch, nullptr, info, passes, max_passes);
+}
+
+int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                                  const afft_gpu_variant* v, afft_gpu_plan_info* info, afft_gpu_pass_info* passes,
+                                  int max_passes) {
   idx_t d[kMaxDim];
   int nd = 0;
   if (validate(ndim, dim, format, afft_gpu, d, nd) != 0) return -1;
@@ -1936,6 +1976,17 @@ int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* di
   P->ndim = nd;
   for (int i = 0; i < nd; i++) P->dims[i] = d[i];
   P->batch = (idx_t)batch;
+  if (v) {
+    P->var.col_tile = v->col_tile;
+    P->var.rowt_tile = v->rowt_tile;
+    P->var.stream_io = v->stream_io;
+    P->var.xcd_order = v->xcd_order;
+    P->var.chunk = v->chunk;
+    P->var.sched = v->sched;
+    P->var.half_lds = v->half_lds;
+    P->var.pipeline = v->pipeline;
+    P->var.real_pair = v->real_pair;
+  }
   g_dry = true;
   bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)
                             : Builder<double>(*P).build(kind, format, d, nd);

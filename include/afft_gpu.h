@@ -108,7 +108,9 @@ typedef struct
   int32_t real_pair; /* fused real passes.  2-D r2c / c2r with dim0 > 4096 and a
                         1024..8192-point last axis: the real pass over row pairs
                         (k, h-k) runs inside the row FFTs (0: rows up to 4096
-                        points, 1: all, -1: a pass of its own).  FP32 1-D r2c,
+                        points, 1: all, -1: a pass of its own).  FP32 1-D c2r with
+                        2^20 <= N <= 2^22: the inverse real pre-pass runs in the
+                        load of four-step pass A (0 or 1; -1: a pass of its own).  FP32 1-D r2c,
                         2^16 <= N <= 2^24: 1 runs pass B of the four-step with the
                         real post-pass and the output transposition fused in (rows
                         k1 and n1-k1 per work-group; half the scratch, measured
@@ -232,6 +234,11 @@ typedef struct
  * check a plan's addressing without a GPU. */
 int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
                           afft_gpu_plan_info* info, afft_gpu_pass_info* passes, int max_passes);
+/* The same for the plan a variant (afft_gpu_plan_set_variant) would build;
+   v = NULL: the default variant. */
+int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                                  const afft_gpu_variant* v, afft_gpu_plan_info* info, afft_gpu_pass_info* passes,
+                                  int max_passes);
 
 /* ---- helpers -------------------------------------------------------------- */
 /* acc[0] = sum (y[i] - ref[i])^2, acc[1] = sum ref[i]^2 over n values, in

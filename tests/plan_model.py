@@ -44,16 +44,25 @@ class PlanInfo(ctypes.Structure):
 KIND = {"c": 0, "ci": 1, "r": 2, "ri": 3}
 
 
-def inspect(kind: str, dims, fmt: int, batch: int = 1, prec: int = 32):
+def inspect(kind: str, dims, fmt: int, batch: int = 1, prec: int = 32, variant: dict | None = None):
+    """Pass list of the plan (variant: afft_gpu_variant fields, others 0 /
+    stream_io 1 as afft.Plan.set_variant sets them)."""
+    from fft_amd.afft import Variant
     L = _lib.lib()
-    f = L.afft_gpu_plan_inspect
+    f = L.afft_gpu_plan_inspect_variant
     f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.c_int,
-                  ctypes.c_size_t, ctypes.POINTER(PlanInfo), ctypes.POINTER(PassInfo), ctypes.c_int]
+                  ctypes.c_size_t, ctypes.c_void_p, ctypes.POINTER(PlanInfo), ctypes.POINTER(PassInfo), ctypes.c_int]
     f.restype = ctypes.c_int
     nd, arr = _lib.dims_array(dims)
     info = PlanInfo()
     passes = (PassInfo * 200)()
-    n = f(prec, KIND[kind], nd, arr, fmt, batch, ctypes.byref(info), passes, 200)
+    v = None
+    if variant is not None:
+        v = Variant(stream_io=1)
+        for k, val in variant.items():
+            setattr(v, k, int(val))
+    n = f(prec, KIND[kind], nd, arr, fmt, batch, ctypes.byref(v) if v is not None else None, ctypes.byref(info),
+          passes, 200)
     if n < 0:
         raise ValueError(_lib.last_error())
     return info, [passes[i] for i in range(n)]
@@ -155,6 +164,15 @@ def _fft_pass(p, bufs, start, cnt, pi):
     j = np.arange(nin).reshape(1, 1, nin)
     e_in = _addr(p.in_, start, b, p.n_blo, c, j)
     x = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    if p.epi == 4:  # c2r pre-pass pairing bin k with bin h - k (another column)
+        h = p.rr_h
+        k = c * p.in_.s_c + j * p.in_.s_j
+        s0 = x
+        s1 = _gather(p.in_, bufs, e_in - k + (h - k), f"pass {pi} load (pair)")
+        w = np.exp(-2j * np.pi * k / (2 * h))
+        z = (s0 + np.conj(s1)) - 1j * (np.conj(s1) - s0) * np.conj(w)
+        z0 = (s0.real + s1.real) + 1j * (s0.real - s1.real)
+        x = np.where(k == 0, z0, z)
     if p.epi == 3:  # c2r pre-pass
         s0 = x[..., :L]
         s1 = x[..., L:0:-1]  # S[L - k], k = 0..L-1
@@ -168,7 +186,7 @@ def _fft_pass(p, bufs, start, cnt, pi):
     if p.swap_in:
         x = _swap(x)
     X = np.fft.fft(x, axis=-1)
-    if p.epi == 1:  # four-step twiddle
+    if p.epi in (1, 4):  # four-step twiddle
         if p.tw4_blo:
             n2 = (np.arange(nb) % p.n_blo).reshape(nb, 1, 1)
             N = L * p.n_blo
@@ -375,10 +393,11 @@ def _real_irowfft(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, ob + rows.reshape(1, -1, 1) * p.out.s_c + j * p.out.s_j, y, f"pass {pi} store")
 
 
-def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None):
+def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant=None):
     """Executes the plan model; returns the dst arrays (float64).  edit
-    (optional) maps the inspected pass list to the one executed."""
-    info, passes = inspect(kind, dims, fmt, batch, prec)
+    (optional) maps the inspected pass list to the one executed; variant
+    (optional) selects the plan a kernel-shape variant builds."""
+    info, passes = inspect(kind, dims, fmt, batch, prec, variant)
     if edit is not None:
         passes = edit(passes)
     bufs = Buffers(kind, dims, fmt, batch, info)

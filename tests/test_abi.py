@@ -98,41 +98,25 @@ def _unpack(re_, im_, fmt):
     return re_[0::2] + 1j * re_[1::2] if fmt == IL else re_ + 1j * im_
 
 
-def fuse_real_pass(passes):
-    """The pass list of the fused-real-pass variant (afft_gpu_variant.real_pair
-    = 1) from the default one: the row pass B (type 0) and the transposing
-    real pass (type 3) become one type-4 pass with B's input and the real
-    pass's output, 16 rows (8 pairs) per work-group."""
-    assert [p.type for p in passes[-2:]] == [0, 3], [p.type for p in passes]
-    b, tr = passes[-2], passes[-1]
-    f = M.PassInfo()
-    ctypes.pointer(f)[0] = tr
-    f.type = 4
-    f.L = b.L
-    f.CT = 16
-    f.in_ = b.in_
-    return passes[:-2] + [f]
-
-
-def model_check(kind, dims, fmt, batch, prec=32, edit=None):
+def model_check(kind, dims, fmt, batch, prec=32, variant=None):
     s = O.squeeze_shape(dims)
     n = int(np.prod(s))
     if kind in ("c", "ci"):
         x = O.complex_input(1, (batch, n)).astype(np.complex128)
         re_, im_ = _pack(x, fmt)
-        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec, variant=variant)
         y = _unpack(dre, dim_, fmt).reshape(batch, n)
         ref = np.stack([O.fft(x[b].reshape(s), inverse=(kind == "ci")).ravel() for b in range(batch)])
     elif kind == "r":
         x = O.normal(2, batch * n).astype(np.float64).reshape(batch, n)
-        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, x, None, prec, edit)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, x, None, prec, variant=variant)
         y = _unpack(dre, dim_, fmt).reshape(batch, -1)
         ref = np.stack([O.rfft(x[b].reshape(s)).ravel() for b in range(batch)])
     else:
         hs = O.half_shape(s)
         z = np.stack([O.hermitian_input(3 + b, s).astype(np.complex128).ravel() for b in range(batch)])
         re_, im_ = _pack(z, fmt)
-        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec)
+        dre, dim_, info, passes = M.run(kind, dims, fmt, batch, re_, im_, prec, variant=variant)
         y = dre.reshape(batch, n)
         ref = np.stack([O.irfft(z[b].reshape(hs), s).ravel() for b in range(batch)])
     return np.abs(y - ref).max() / max(1e-300, np.abs(ref).max()), info, passes
@@ -164,13 +148,27 @@ def test_plan_model_real_pass_in_rows(kind):
             assert err < 1e-9, (kind, fmt, batch, err)
 
 
+@pytest.mark.parametrize("lg", [16, 20, 21, 22])
+def test_plan_model_c2r_pre_pass_in_pass_a(lg):
+    # long 1-D c2r, opt-in: the inverse real pre-pass runs in the load of
+    # four-step pass A (pairing bins k and h-k across columns) where a kernel
+    # exists; other shapes and variants keep the separate real pass
+    for fmt in (IL, SP):
+        for v in ({"real_pair": 1}, {"real_pair": 1, "sched": -1}, None):
+            err, info, passes = model_check("ri", (1 << lg,), fmt, 2, variant=v)
+            fused = [p.epi for p in passes] == [4, 0]
+            assert fused == (lg >= 20 and v is not None and "sched" not in v), [(p.type, p.L, p.epi) for p in passes]
+            assert err < 1e-9, (lg, fmt, v, err)
+
+
 @pytest.mark.parametrize("lg", [16, 17, 20, 22])
 def test_plan_model_fused_real_pass(lg):
     # the opt-in fused real pass (rows k1 and n1 - k1 per work-group, 1024- or
     # 2048-point rows): slot mapping, partner bins, row 0 / row n1/2 and the
     # Nyquist bin against the oracle, both formats, a batch of 3
     for fmt in (IL, SP):
-        err, info, passes = model_check("r", (1 << lg,), fmt, 3 if lg < 22 else 1, edit=fuse_real_pass)
+        err, info, passes = model_check("r", (1 << lg,), fmt, 3 if lg < 22 else 1, variant={"real_pair": 1})
+        assert [p.type for p in passes][-1] == 4, [p.type for p in passes]
         assert err < 1e-9, (lg, fmt, err)
 
 

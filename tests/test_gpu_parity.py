@@ -248,7 +248,7 @@ VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chun
 @pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
                                           ("c", 1 << 17, 2), ("c", 1 << 14, 3), ("ci", 1 << 13, 3),
                                           ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 3),
-                                          ("r", 1 << 17, 3)])
+                                          ("r", 1 << 17, 3), ("ri", 1 << 21, 2)])
 def test_kernel_variants_against_oracle(G, kind, n, batch):
     """Every kernel-shape variant (radix-32 schedules, half-LDS exchange,
     narrow tiles) against the FP64 oracle; the variants only change speed."""
