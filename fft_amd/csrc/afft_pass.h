@@ -67,6 +67,10 @@ struct PassParams {
   int swap_in, swap_out;
   int in_nt, out_nt;  // streaming (non-temporal) loads / stores
   int xcd_remap;      // give each XCD a contiguous run of tiles
+  int mirror_pairs;   // EPI_C2R4: n_tiles counts whole groups of 16 blocks; blocks
+                      // 16q + x and 16q + 8 + x run column tile c and its mirror
+                      // col_tiles - 1 - c of one batch entry on the same XCD
+  int n_batch;        // mirror_pairs: batch entries (tiles = n_batch * col_tiles)
   const cx<T>* stage_tw;  // per-length stage twiddles
   const cx<T>* tw4_hi;    // W_N^(i << lo_bits)
   const cx<T>* tw4_lo;    // W_N^j - 1, j < 2^lo_bits
@@ -663,7 +667,21 @@ template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FA
 __global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (pass_min_waves<T, L, CT, COLMAP, STORE_T, SV, HX>()))
     fft_pass_kernel(PassParams<T> p) {
   extern __shared__ __align__(16) unsigned char afgpu_smem[];
-  const int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
+  int tile = p.xcd_remap ? xcd_tile(blockIdx.x, p.n_tiles) : (int)blockIdx.x;
+  if constexpr (EPI == EPI_C2R4) {
+    // the partner bins of column tile c lie in its mirror tile: run both on
+    // one XCD (blocks go round-robin over the 8 XCDs) so that the second read
+    // of every bin hits that XCD's L2
+    if (p.mirror_pairs) {
+      const int B = blockIdx.x;
+      const int pair = (B >> 4) * 8 + (B & 7);
+      const int half = p.col_tiles / 2;
+      const int b = pair / half;
+      if (b >= p.n_batch) return;
+      const int pp = pair - b * half;
+      tile = b * p.col_tiles + ((B & 8) ? p.col_tiles - 1 - pp : pp);
+    }
+  }
   fft_tile<T, L, CT, COLMAP, EPI, STORE_T, FAST, SV, HX>(p, tile, threadIdx.x,
                                                          reinterpret_cast<cx<T>*>(afgpu_smem));
 }

@@ -1118,6 +1118,12 @@ bool fft_params(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, Pass
   p.in_nt = r.in_nt;
   p.out_nt = r.out_nt;
   p.xcd_remap = r.xcd;
+  if (r.epi == EPI_C2R4 && tiles >= 2 && tiles % 2 == 0 && !r.xcd) {
+    // mirror tiles paired on one XCD (see PassParams::mirror_pairs)
+    p.mirror_pairs = 1;
+    p.n_batch = (int)nb;
+    p.n_tiles = (int)((nb * (tiles / 2) + 7) / 8 * 16);
+  }
   p.stage_tw = (const cx<T>*)r.stage_tw;
   p.tw4_hi = (const cx<T>*)r.tw4_hi;
   p.tw4_lo = (const cx<T>*)r.tw4_lo;

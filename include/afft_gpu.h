@@ -109,8 +109,9 @@ typedef struct
                         1024..8192-point last axis: the real pass over row pairs
                         (k, h-k) runs inside the row FFTs (0: rows up to 4096
                         points, 1: all, -1: a pass of its own).  FP32 1-D c2r with
-                        2^20 <= N <= 2^22: the inverse real pre-pass runs in the
-                        load of four-step pass A (0 or 1; -1: a pass of its own).  FP32 1-D r2c,
+                        2^20 <= N <= 2^22: 1 runs the inverse real pre-pass in the
+                        load of four-step pass A (measured slower); 0 or -1: a
+                        pass of its own.  FP32 1-D r2c,
                         2^16 <= N <= 2^24: 1 runs pass B of the four-step with the
                         real post-pass and the output transposition fused in (rows
                         k1 and n1-k1 per work-group; half the scratch, measured
