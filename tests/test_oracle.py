@@ -160,7 +160,7 @@ def test_oracle_matches_afft_large_summaries():
         kind, shape, fmt, isa = _parse(k)
         n = int(np.prod(shape))
         if n > (1 << 20):
-            continue  # larger shapes are compared on the GPU box against the device FP64 path
+            continue  # larger shapes are compared on the GPU against the device FP64 path
         if kind == "c":
             ref = O.fft(O.complex_input(1, shape).astype(np.complex128))
         else:
