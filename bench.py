@@ -292,7 +292,7 @@ def main():
 
     extra = {"step_wall_ms": round(wall, 3), "step_hbm_frac_algorithmic": round(hbm_frac_step, 4),
              "pass_a_ms_per_direction": round(ms_a, 4), "pass_b_ms_per_direction": round(ms_b, 4),
-             "chunk": int(batch / launches)}
+             "chunk": int(fwd.get_variant().chunk), "pass_a_launch_transforms": int(batch / launches)}
     # measured copy ceiling on the same buffers
     ms_copy = time_events(torch, stream, lambda: _lib.lib().afft_gpu_copy(int(x.data_ptr()), int(z.data_ptr()),
                                                                          x.numel() * 4, stream.cuda_stream), 10)

@@ -128,6 +128,7 @@ struct Variant {
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
   int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
   int real_pair = 0;  // fused real passes: 1 = all (also 1-D r2c pass B), 0 (default) = the N-D row-pair one, -1 = none
+  int a_split = 0;    // two-pass plans: launches of pass A per group (0 / 1: one)
 };
 
 struct Plan {
@@ -1301,6 +1302,41 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
   return true;
 }
 
+// Launches per group for pass A of a two-pass plan (variant a_split): pass A
+// reads the caller's input into the scratch and pass B reads only the scratch,
+// so pass A can run the group in parts (each writing its part of the scratch)
+// while pass B runs it whole.
+bool split_shape(const Plan& P) {
+  if (P.npass != 2) return false;
+  const PassRec& a = P.pass[0];
+  const PassRec& b = P.pass[1];
+  if (a.type != PT_FFT || a.mfma || a.fold_user || a.in.role >= R_SCR || a.out.role < R_SCR) return false;
+  return b.in.role == a.out.role && b.out.role < R_SCR;
+}
+
+// Default (a_split = 0) for four-step plans whose pass A has long (>= 1024-
+// point) columns: pass A in groups of 64 MiB of scratch (which stay in the
+// MALL between its launches), pass B over four of them at once (fewer
+// launches, several waves per launch).  64 x 2^20 c2c: 0.44 vs 0.48-0.51 ms
+// per direction; 32 x 2^21: 0.52 vs 0.57.
+constexpr int kAutoSplit = 4;
+bool auto_split(const Plan& P) {
+  return P.var.a_split == 0 && P.var.chunk == 0 && !P.var.pipeline && split_shape(P) && P.pass[0].colmap &&
+         P.pass[0].L >= 1024;
+}
+
+int a_split(const Plan& P) {
+  if (P.pipe || !split_shape(P)) return 1;
+  if (P.var.a_split > 1) return P.var.a_split;
+  if (!auto_split(P)) return 1;
+  // pass A launches of one 64 MiB group each
+  const idx_t per = P.scr_per_user[0] > P.scr_per_user[1] ? P.scr_per_user[0] : P.scr_per_user[1];
+  const idx_t eb = P.prec == 32 ? 8 : 16;
+  const idx_t g = per > 0 ? ((idx_t)64 << 20) / (per * eb) : 0;
+  if (g < 1 || P.chunk % g != 0) return 1;
+  return (int)(P.chunk / g);
+}
+
 template <typename T>
 bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
              int last = -1) {
@@ -1347,10 +1383,24 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
     }
     return true;
   }
+  const int split = a_split(P);
   for (idx_t start = 0; start < P.batch; start += P.chunk) {
     idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
-    for (int i = first; i < last; i++)
+    for (int i = first; i < last; i++) {
+      if (i == 0 && split > 1 && cnt % split == 0) {
+        // pass A of the group in `split` launches, each into its part of the
+        // scratch (the scratch holds the group's transforms back to back)
+        const idx_t sub = cnt / split;
+        for (int q = 0; q < split; q++) {
+          Bufs<T> Bq = B;
+          const int role = P.pass[0].out.role - R_SCR;
+          Bq.scr[role] = B.scr[role] + 2 * (size_t)(q * sub) * (size_t)P.scr_per_user[role];
+          if (!run_pass<T>(P.pass[0], Bq, start + q * sub, sub, st)) return false;
+        }
+        continue;
+      }
       if (!run_pass<T>(P.pass[i], B, start, cnt, st)) return false;
+    }
   }
   return true;
 }
@@ -1385,6 +1435,8 @@ idx_t default_chunk(const Plan& P, size_t elem_bytes) {
   // measured faster than keeping the scratch within the MALL
   const bool short_cols = P.npass >= 1 && P.pass[0].type == PT_FFT && P.pass[0].colmap && P.pass[0].L <= 512;
   idx_t target = (idx_t)((short_cols ? 256u : 64u) << 20) / (idx_t)(per * elem_bytes);
+  // pass A split into launches of the 64 MiB groups, pass B over kAutoSplit of them
+  if (!short_cols && auto_split(P) && target >= 1) target *= kAutoSplit;
   if (target < 1) target = 1;
   return target < P.batch ? target : P.batch;
 }
@@ -1526,7 +1578,8 @@ TuneCache& tcache() {
 }
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
-//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline> [<real_pair>]]]
+//   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline> [<real_pair>
+//   [<a_split>]]]]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1538,8 +1591,9 @@ void tcache_load_locked(TuneCache& c) {
   while (fgets(line, sizeof(line), f)) {
     Variant v;
     long long ch = 0;
-    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d", key, &v.col_tile, &v.rowt_tile, &v.stream_io,
-                           &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline, &v.real_pair);
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d %d", key, &v.col_tile, &v.rowt_tile,
+                           &v.stream_io, &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline, &v.real_pair,
+                           &v.a_split);
     if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
@@ -1566,8 +1620,8 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
-          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair);
+  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
+          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair, v.a_split);
   fclose(f);
 }
 
@@ -1790,7 +1844,13 @@ int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass) {
   if (!read_header(plan, h)) return -1;
   const Plan& P = *h.plan;
   if (pass < 0 || pass >= P.npass) return -1;
-  return (int)((P.batch + P.chunk - 1) / P.chunk);
+  const int split = pass == 0 ? a_split(P) : 1;
+  int n = 0;
+  for (idx_t start = 0; start < P.batch; start += P.chunk) {
+    const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
+    n += (split > 1 && cnt % split == 0) ? split : 1;
+  }
+  return n;
 }
 
 void afft_gpu_plan_destroy(afft_gpu_plan* plan) { destroy_impl(plan); }
@@ -1835,6 +1895,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.half_lds = v->half_lds;
   P.var.pipeline = v->pipeline;
   P.var.real_pair = v->real_pair;
+  P.var.a_split = v->a_split;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1885,6 +1946,8 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->half_lds = P.var.half_lds;
   v->pipeline = P.var.pipeline;
   v->real_pair = P.var.real_pair;
+  v->a_split = P.var.a_split;
+  v->pad_ = 0;
   return 0;
 }
 
@@ -1992,6 +2055,7 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
     P->var.half_lds = v->half_lds;
     P->var.pipeline = v->pipeline;
     P->var.real_pair = v->real_pair;
+    P->var.a_split = v->a_split;
   }
   g_dry = true;
   bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)

@@ -117,6 +117,10 @@ typedef struct
                         k1 and n1-k1 per work-group; half the scratch, measured
                         5-10 % slower); 0 (default) or -1: a row pass into a second
                         scratch, then the transposing real pass */
+  int32_t a_split;   /* two-pass plans (pass A: input -> scratch, pass B: scratch ->
+                        output): pass A of each group runs in this many launches,
+                        pass B in one (0 or 1: one) */
+  int32_t pad_;
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);

@@ -302,6 +302,33 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
     p.close()
 
 
+@pytest.mark.parametrize("kind,n,batch,chunk,split", [("c", 1 << 20, 8, 8, 2), ("ci", 1 << 20, 12, 6, 3),
+                                                      ("c", 1 << 16, 64, 32, 4), ("r", 1 << 20, 4, 4, 2)])
+def test_split_pass_a_matches_one_launch(G, kind, n, batch, chunk, split):
+    """Pass A of each group in several launches (variant a_split) computes
+    exactly what one launch does: same kernels, same scratch layout."""
+    import torch
+    from fft_amd.afft import Plan
+    say("a_split", kind, n, batch, chunk, split)
+    if kind == "r":
+        src = G.to_dev(O.normal(61, batch * n))
+        nout = 2 * (n // 2 + 1) * batch
+    else:
+        x = O.complex_input(61, (batch, n))
+        src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+        nout = 2 * n * batch
+    p = Plan(kind, (n,), batch)
+    p.set_variant(chunk=chunk)
+    ref = G.empty_dev(nout)
+    p(src, None, ref, None)
+    p.set_variant(chunk=chunk, a_split=split)
+    out = G.empty_dev(nout)
+    p(src, None, out, None)
+    torch.cuda.synchronize()
+    assert torch.equal(ref, out)
+    p.close()
+
+
 def test_pipelined_plan_graph_replay(G):
     """A four-step plan over several groups runs pass B of one group on the
     plan's second stream while pass A of the next runs on the caller's; the
