@@ -129,6 +129,7 @@ struct Variant {
   int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
   int real_pair = 0;  // fused real passes: 1 = all (also 1-D r2c pass B), 0 (default) = the N-D row-pair one, -1 = none
   int a_split = 0;    // two-pass plans: launches of pass A per group (0 / 1: one)
+  int row_log2 = 0;   // 1-D four-step: row length n2 = 2^row_log2 (0 = default split)
 };
 
 struct Plan {
@@ -267,6 +268,14 @@ struct Builder {
     }
   }
 
+  // Row length n2 of a 1-D four-step n = n1 * n2: 1024-point rows up to 2^20;
+  // beyond, 2048-point rows keep the column pass at <= 2048 points (2^21 as
+  // 1024 x 2048 measured 3-5 % faster than 2048 x 1024); a variant may set it.
+  idx_t row_len_1d(idx_t n) const {
+    if (P.var.row_log2 > 0) return P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : 0;
+    return n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
+  }
+
   // pref_ct > 0: a tile tried before the defaults (split-addressed column
   // passes ask for wide tiles, whose 4-byte re / im accesses then still cover
   // whole 256-byte row segments)
@@ -278,11 +287,14 @@ struct Builder {
     const int want = colmap ? P.var.col_tile : (store_t ? P.var.rowt_tile : 0);
     const PassVariant<T>* v = nullptr;
     // column passes without an explicit tile first try a half-LDS kernel with
-    // twice the default tile (wider row segments at the same LDS footprint);
-    // short columns (<= 128 points) a 4x-wide tile, whose LDS is still small
-    // (not for 1024-point columns: 32-column tiles, one group per CU, measured
-    // 0.285 against 0.255 ms per 64 x 2^20 direction for two 16-column groups;
-    // they stay a variant, col_tile = 32)
+    // twice the default tile (wider row segments at the same LDS footprint;
+    // compiled for the 256- and 512-point four-step pass A, where it measured
+    // 5-8 % faster; a plain 256-point 32-column half-LDS tile measured 3 %
+    // slower on 256^3 and is not compiled); short columns (<= 128 points) a
+    // 4x-wide tile, whose LDS is still small (not for 1024-point columns:
+    // 32-column tiles, one group per CU, measured 0.285 against 0.255 ms per
+    // 64 x 2^20 direction for two 16-column groups; they stay a variant,
+    // col_tile = 32)
     const bool wide = want <= 0 && colmap && P.var.half_lds >= 0 && L != 1024;
     if (pref_ct <= 0 && want <= 0 && colmap && L <= 128) pref_ct = 4 * deflt;
     struct Cand {
@@ -396,12 +408,9 @@ struct Builder {
     // four-step along the axis: n = n1 * n2
     idx_t n2, n1;
     if (inner == 1) {
-      // 1024-point rows up to 2^20; beyond, 2048-point rows keep the column
-      // pass at <= 2048 points (2^21 as 1024 x 2048 measured 3-5 % faster
-      // than 2048 x 1024)
-      n2 = n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
-      if (P.mode != afft_gpu_mode_fma && n == 65536) n2 = 256;  // 256 x 256 on the matrix cores
-      if (n2 > 4096 || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
+      n2 = row_len_1d(n);
+      if (P.mode != afft_gpu_mode_fma && n == 65536 && P.var.row_log2 <= 0) n2 = 256;  // 256 x 256 on the matrix cores
+      if (n2 < 2 || n2 > 4096 || n2 >= n || !find_pass<T>((int)n2, 0, 0, EPI_PLAIN, 1)) {
         set_err("transform length %lld is too long", (long long)n);
         ok = false;
         return;
@@ -514,8 +523,8 @@ struct Builder {
   // bins in natural order (one kernel for the transposition and the real
   // post-pass instead of a transposed store and a separate real pass).
   void r2c_fourstep_tr(const Side& in, const Side& out, idx_t h) {
-    const idx_t n2 = h <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, h / max_col_len<T>());
-    const idx_t n1 = h / n2;
+    const idx_t n2 = row_len_1d(h);
+    const idx_t n1 = n2 >= 2 ? h / n2 : 0;
     if (n1 > max_col_len<T>() || n1 < 32) {
       set_err("transform length %lld is too long", (long long)(2 * h));
       ok = false;
@@ -840,8 +849,8 @@ struct Builder {
           r->fold_user = 1;
           return ok;
         }
-        const idx_t n2 = h <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, h / max_col_len<T>());
-        if (h / n2 >= 32) {
+        const idx_t n2 = row_len_1d(h);
+        if (n2 >= 2 && h / n2 >= 32) {
           r2c_fourstep_tr(in, out, h);
           return ok;
         }
@@ -1896,6 +1905,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.pipeline = v->pipeline;
   P.var.real_pair = v->real_pair;
   P.var.a_split = v->a_split;
+  P.var.row_log2 = v->row_log2;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -1947,7 +1957,7 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->pipeline = P.var.pipeline;
   v->real_pair = P.var.real_pair;
   v->a_split = P.var.a_split;
-  v->pad_ = 0;
+  v->row_log2 = P.var.row_log2;
   return 0;
 }
 
@@ -2056,6 +2066,7 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
     P->var.pipeline = v->pipeline;
     P->var.real_pair = v->real_pair;
     P->var.a_split = v->a_split;
+    P->var.row_log2 = v->row_log2;
   }
   g_dry = true;
   bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)

@@ -92,8 +92,9 @@ int afft_gpu_mfma_selftest(void);
  * they differ in speed.  0 means "default" for the integer fields. */
 typedef struct
 {
-  int32_t col_tile;  /* columns per work-group of column passes with L >= 512 (8, 16; 32 for
-                        the 1024-point four-step columns of FP32 plans) */
+  int32_t col_tile;  /* columns per work-group of column passes (8, 16; in FP32 plans 32
+                        for the 256-, 512- and 1024-point and 64 for the 256-point
+                        four-step columns) */
   int32_t rowt_tile; /* rows per work-group of transposed-store row passes (4, 8 or 16) */
   int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
@@ -120,7 +121,10 @@ typedef struct
   int32_t a_split;   /* two-pass plans (pass A: input -> scratch, pass B: scratch ->
                         output): pass A of each group runs in this many launches,
                         pass B in one (0 or 1: one) */
-  int32_t pad_;
+  int32_t row_log2;  /* 1-D four-step plans: row length n2 = 2^row_log2 of the split
+                        n = n1 * n2 (pass A: n2 columns of n1 points, pass B: n1 rows
+                        of n2 points); 0 (default): 1024 up to 2^20, longer rows
+                        beyond */
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);

@@ -172,6 +172,22 @@ def test_plan_model_fused_real_pass(lg):
         assert err < 1e-9, (lg, fmt, err)
 
 
+@pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])
+def test_plan_model_row_split_variant(kind):
+    # other splits n = n1 * n2 of the 1-D four-step (variant row_log2): pass B
+    # rows of 2^row_log2 points (r2c / c2r split their half-length complex
+    # transform the same way)
+    for lg, rl in ((17, 9), (17, 11), (20, 8), (20, 11), (20, 12)):
+        err, info, passes = model_check(kind, (1 << lg,), IL, 2, variant={"row_log2": rl})
+        ffts = [p for p in passes if p.type == 0]
+        assert ffts[-1].L == 1 << rl and ffts[0].colmap == 1, [(p.type, p.L) for p in passes]
+        assert err < 1e-9, (kind, lg, rl, err)
+    # a row length that does not split n: the plan is refused
+    for rl in (3, 20, 30):
+        with pytest.raises(ValueError):
+            M.inspect(kind, (1 << 20,), IL, 1, variant={"row_log2": rl})
+
+
 def test_plan_model_fp64_and_chunking():
     # a batch larger than one chunk exercises the per-chunk scratch reuse
     info, passes = M.inspect("c", (1 << 20,), IL, 64)

@@ -245,17 +245,9 @@ VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chun
             for rp in (1, -1)]
 
 
-@pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
-                                          ("c", 1 << 17, 2), ("c", 1 << 14, 3), ("ci", 1 << 13, 3),
-                                          ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 3),
-                                          ("r", 1 << 17, 3), ("ri", 1 << 21, 2)])
-def test_kernel_variants_against_oracle(G, kind, n, batch):
-    """Every kernel-shape variant (radix-32 schedules, half-LDS exchange,
-    narrow tiles) against the FP64 oracle; the variants only change speed."""
-    import torch
-    from fft_amd.afft import Plan
-    say("variants", kind, n, batch)
-    real = kind in ("r", "ri")
+def _variant_case(G, kind, n, batch):
+    """Seeded input on the device, the FP64 oracle's outputs and a function
+    that checks one plan's device output against them."""
     if kind == "r":
         x = O.normal(31, batch * n).reshape(batch, n).astype(np.float32)
         ref = [O.rfft(x[b].astype(np.float64)) for b in range(batch)]
@@ -271,15 +263,9 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
         ref = [O.fft(x[b].astype(np.complex128), inverse=(kind == "ci")) for b in range(batch)]
         src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
         nout = 2 * n * batch
-    p = Plan(kind, (n,), batch)
-    seen = set()
-    for v in VARIANTS:
-        p.set_variant(**v)
-        desc = tuple(p.describe())
-        if desc in seen:
-            continue
-        seen.add(desc)
-        say("  variant", v, desc)
+
+    def check(p, v):
+        import torch
         dst = G.empty_dev(nout)
         p(src, None, dst, None)
         torch.cuda.synchronize()
@@ -293,12 +279,63 @@ def test_kernel_variants_against_oracle(G, kind, n, batch):
         for b in range(batch):
             yb = y[b].astype(np.float32 if kind == "ri" else np.complex64)
             assert O.rel_l2(yb, ref[b]) <= O.bound(n), (kind, n, v, b)
+
+    return check
+
+
+@pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 3), ("ci", 1 << 20, 2), ("c", 1 << 22, 1),
+                                          ("c", 1 << 17, 2), ("c", 1 << 14, 3), ("ci", 1 << 13, 3),
+                                          ("r", 1 << 14, 2), ("ri", 1 << 14, 2), ("r", 1 << 21, 3),
+                                          ("r", 1 << 17, 3), ("ri", 1 << 21, 2)])
+def test_kernel_variants_against_oracle(G, kind, n, batch):
+    """Every kernel-shape variant (radix-32 schedules, half-LDS exchange,
+    narrow tiles) against the FP64 oracle; the variants only change speed."""
+    from fft_amd.afft import Plan
+    say("variants", kind, n, batch)
+    check = _variant_case(G, kind, n, batch)
+    p = Plan(kind, (n,), batch)
+    seen = set()
+    for v in VARIANTS:
+        p.set_variant(**v)
+        desc = tuple(p.describe())
+        if desc in seen:
+            continue
+        seen.add(desc)
+        say("  variant", v, desc)
+        check(p, v)
     if kind in ("c", "ci") and n > (1 << 14):
         assert len(seen) >= 2, seen  # four-step plans do have alternatives
     if kind == "r" and n >= (1 << 16):
         # the fused real pass and the row pass + transposing real pass
         assert any("real_pair" in d for s in seen for d in s), seen
         assert any("real_tr" in d for s in seen for d in s), seen
+    p.close()
+
+
+@pytest.mark.parametrize("kind,n,batch", [("c", 1 << 20, 2), ("ci", 1 << 20, 1), ("r", 1 << 20, 2),
+                                          ("ri", 1 << 20, 1), ("c", 1 << 19, 2), ("c", 1 << 18, 2)])
+def test_row_split_variants_against_oracle(G, kind, n, batch):
+    """Other splits n = n1 * n2 of the 1-D four-step (variant row_log2) and
+    the wide 256- / 512-point pass-A tiles (32 and 64 columns, half LDS)
+    against the FP64 oracle."""
+    from fft_amd.afft import Plan
+    say("row split", kind, n, batch)
+    check = _variant_case(G, kind, n, batch)
+    p = Plan(kind, (n,), batch)
+    seen = set()
+    for rl in (0, 8, 9, 11, 12):
+        for ct in (0, 16, 32, 64):
+            try:
+                p.set_variant(row_log2=rl, col_tile=ct)
+            except RuntimeError:
+                continue  # no kernel of that tile for this split
+            desc = tuple(p.describe())
+            if desc in seen:
+                continue
+            seen.add(desc)
+            say("  variant", rl, ct, desc)
+            check(p, (rl, ct))
+    assert len(seen) >= 4, seen
     p.close()
 
 
