@@ -1311,27 +1311,37 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
   return true;
 }
 
-// Launches per group for pass A of a two-pass plan (variant a_split): pass A
-// reads the caller's input into the scratch and pass B reads only the scratch,
-// so pass A can run the group in parts (each writing its part of the scratch)
-// while pass B runs it whole.
+// Launches per group for pass A of a four-step plan (variant a_split): pass A
+// reads the caller's input into a scratch that holds the group's transforms
+// back to back, and the later passes (pass B; for 1-D r2c also the
+// transposing real pass) start from that scratch, so pass A can run the group
+// in parts (each writing its part of the scratch) while the others run it
+// whole.
 bool split_shape(const Plan& P) {
-  if (P.npass != 2) return false;
+  if (P.npass < 2) return false;
   const PassRec& a = P.pass[0];
   const PassRec& b = P.pass[1];
   if (a.type != PT_FFT || a.mfma || a.fold_user || a.in.role >= R_SCR || a.out.role < R_SCR) return false;
-  return b.in.role == a.out.role && b.out.role < R_SCR;
+  if (a.b_per_user != 1 || a.n_blo > 1 || a.out.s_b != P.scr_per_user[a.out.role - R_SCR]) return false;
+  return b.in.role == a.out.role && P.pass[P.npass - 1].out.role < R_SCR;
 }
 
 // Default (a_split = 0) for four-step plans whose pass A has long (>= 1024-
 // point) columns: pass A in groups of 64 MiB of scratch (which stay in the
 // MALL between its launches), pass B over four of them at once (fewer
 // launches, several waves per launch).  64 x 2^20 c2c: 0.44 vs 0.48-0.51 ms
-// per direction; 32 x 2^21: 0.52 vs 0.57.
+// per direction; 32 x 2^21: 0.52 vs 0.57.  With the transposing real pass
+// after pass B (1-D r2c) only where a group holds at most 4 transforms:
+// 16 x r2c 2^22 0.42 -> 0.40 ms, while 32 x r2c 2^21 measured 0.408 against
+// 0.402 ms and keeps its groups of 8.
 constexpr int kAutoSplit = 4;
 bool auto_split(const Plan& P) {
-  return P.var.a_split == 0 && P.var.chunk == 0 && !P.var.pipeline && split_shape(P) && P.pass[0].colmap &&
-         P.pass[0].L >= 1024;
+  if (P.var.a_split != 0 || P.var.chunk != 0 || P.var.pipeline || !split_shape(P) || !P.pass[0].colmap ||
+      P.pass[0].L < 1024)
+    return false;
+  if (P.npass == 2) return true;
+  const idx_t eb = P.prec == 32 ? 8 : 16;
+  return P.npass == 3 && P.pass[2].type == PT_REALTR && P.scr_per_user[0] * eb >= ((idx_t)16 << 20);
 }
 
 int a_split(const Plan& P) {

@@ -340,7 +340,8 @@ def test_row_split_variants_against_oracle(G, kind, n, batch):
 
 
 @pytest.mark.parametrize("kind,n,batch,chunk,split", [("c", 1 << 20, 8, 8, 2), ("ci", 1 << 20, 12, 6, 3),
-                                                      ("c", 1 << 16, 64, 32, 4), ("r", 1 << 20, 4, 4, 2)])
+                                                      ("c", 1 << 16, 64, 32, 4), ("r", 1 << 20, 4, 4, 2),
+                                                      ("r", 1 << 22, 8, 8, 4)])
 def test_split_pass_a_matches_one_launch(G, kind, n, batch, chunk, split):
     """Pass A of each group in several launches (variant a_split) computes
     exactly what one launch does: same kernels, same scratch layout."""
@@ -359,6 +360,7 @@ def test_split_pass_a_matches_one_launch(G, kind, n, batch, chunk, split):
     ref = G.empty_dev(nout)
     p(src, None, ref, None)
     p.set_variant(chunk=chunk, a_split=split)
+    assert p.launches(0) == (batch // chunk) * split
     out = G.empty_dev(nout)
     p(src, None, out, None)
     torch.cuda.synchronize()
