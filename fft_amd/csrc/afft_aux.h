@@ -96,6 +96,30 @@ struct RealIRowFftArgs {
   const cx<T>* real_tw;   // W_{2h}^k, k < h
 };
 
+// N-D r2c with the halved dim 0 as a column four-step h = h1 * L: pass B
+// (the L-point FFTs over j of Y[k1][j][c], bins k1 + h1*k2) with the real
+// pass over row pairs (k, h-k) fused in (afft_realpair.hip).  Y is the plan's
+// scratch (interleaved, entry (k1, j, c) at k1*L*m + j*m + c of each batch
+// entry); the output holds rows 0..h of m entries (row r at r*m; complex
+// units when out_il, scalar otherwise).
+template <typename T>
+struct RealColPairArgs {
+  const cx<T>* in;
+  T* out_re;
+  T* out_im;
+  int out_il;
+  int L;                  // points of the column FFTs
+  int h1;                 // rows k1 of the four-step
+  idx_t m;                // columns
+  idx_t in_bs, out_bs;    // batch strides (complex / output elements)
+  idx_t nbatch;
+  const cx<T>* stage_tw;  // stage twiddles of the L-point columns (radix-16 schedule)
+  const cx<T>* real_tw;   // W_{2h}^k, k < h
+};
+
+// Columns per work-group of real_colpair (0: no kernel for these shapes).
+int real_colpair_ct(int elem_bytes, int L, idx_t m, int h1);
+
 // all = false: only the row lengths where the fused pass is the default
 bool real_rowfft_supported(int elem_bytes, idx_t h, int L, bool all);
 
@@ -118,6 +142,8 @@ template <typename T>
 hipError_t real_rowfft(const RealRowFftArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_irowfft(const RealIRowFftArgs<T>& a, hipStream_t st);
+template <typename T>
+hipError_t real_colpair(const RealColPairArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

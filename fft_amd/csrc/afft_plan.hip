@@ -76,7 +76,8 @@ enum PassType {
   PT_REALTR = 3,
   PT_REALPAIR = 4,
   PT_REALROWFFT = 5,
-  PT_REALIROWFFT = 6
+  PT_REALIROWFFT = 6,
+  PT_REALCOLPAIR = 7
 };
 
 struct Side {
@@ -659,6 +660,34 @@ struct Builder {
     a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
     a->tw4_bits = bits;
     a->tw4_blo = 1;
+    const int rct = real_pass && P.var.real_pair >= 0 ? real_colpair_ct((int)sizeof(T), (int)h2, inner, (int)h1) : 0;
+    if (rct > 0) {
+      // pass B with the real pass over row pairs (k, h - k): tiles k1 and
+      // h1 - k1 per work-group (afft_realpair.hip)
+      PassRec* r = add();
+      if (!r) return;
+      r->type = PT_REALCOLPAIR;
+      r->L = (int)h2;
+      r->CT = rct;
+      r->colmap = 1;
+      scratch_side(r->in, R_SCR);
+      r->in.s_j = inner;
+      r->in.s_c = 1;
+      r->in.s_b2 = h2 * inner;
+      r->in.s_b = h * inner;
+      r->out = usr_dst;
+      r->out.s_j = inner;
+      r->out.s_c = 1;
+      r->out.s_b = r->out.s_u = (h + 1) * inner;
+      r->n_col = inner;
+      r->n_blo = (int)h1;
+      r->rr_h = h;
+      r->rr_m = inner;
+      r->stage_tw = get_table<T>(0, (int)h2, 0);
+      r->real_tw = get_table<T>(2, h, 0);
+      if (!r->stage_tw || !r->real_tw) ok = false;
+      return;
+    }
     PassRec* b = add();
     if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0)) return;
     scratch_side(b->in, R_SCR);
@@ -864,10 +893,12 @@ struct Builder {
         return ok;
       }
       if (h > kColDirectMax) {
-        // 2-D with a compiled row length: the real pass over row pairs runs
-        // inside the row FFTs (afft_realpair.hip) instead of on its own
+        // the real pass over row pairs runs inside pass B of the dim-0 column
+        // four-step (real_dim0_fourstep_r2c; 8192 x 1024: 0.048 ms, 16384 x
+        // 4096: 0.337 ms) or, on request (real_pair = 1, 2-D with a compiled
+        // row length), inside the row FFTs (0.056 and 0.375 ms)
         const bool fuse =
-            nd == 2 && P.var.real_pair >= 0 && real_rowfft_supported((int)sizeof(T), h, (int)inner, P.var.real_pair > 0);
+            nd == 2 && P.var.real_pair > 0 && real_rowfft_supported((int)sizeof(T), h, (int)inner, true);
         real_dim0_fourstep_r2c(usr_src, usr_dst, n0, inner, !fuse);
         if (fuse) {
           PassRec* r = add();
@@ -1286,6 +1317,23 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     HIP_OK(real_pair<T>(a, st));
     return true;
   }
+  if (r.type == PT_REALCOLPAIR) {
+    RealColPairArgs<T> a{};
+    a.in = reinterpret_cast<const cx<T>*>(ire);
+    a.out_re = ore;
+    a.out_im = oim;
+    a.out_il = r.out.il;
+    a.L = r.L;
+    a.h1 = r.n_blo;
+    a.m = r.rr_m;
+    a.in_bs = r.in.s_b;
+    a.out_bs = r.out.s_b;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(real_colpair<T>(a, st));
+    return true;
+  }
   if (r.type == PT_REALTR) {
     RealTrArgs<T> a{};
     a.in = reinterpret_cast<const cx<T>*>(ire);
@@ -1686,7 +1734,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   // plans with a real pass that has a fused form also try the other form
   for (int i = 0; i < P.npass; i++)
     if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT ||
-        P.pass[i].type == PT_REALIROWFFT) {
+        P.pass[i].type == PT_REALIROWFFT || P.pass[i].type == PT_REALCOLPAIR) {
       Variant v;
       v.real_pair = P.pass[i].type == PT_REALTR ? 1 : -1;
       cands.push_back(v);
@@ -2022,6 +2070,9 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              (long long)r.rr_m, r.out.role, r.out.il);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
+  else if (r.type == PT_REALCOLPAIR)
+    snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
+             (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);
   else
     snprintf(buf, len, "real_one inv=%d", r.rr_inv);
   return 0;

@@ -464,6 +464,140 @@ hipError_t real_irowfft(const RealIRowFftArgs<T>& a, hipStream_t st) {
   }
 }
 
+// ---- column pass B of a dim-0 four-step with the real pass ------------------
+//
+// N-D r2c over a dim 0 of n0 = 2h rows (h = h1 * L): the complex rows Z[r] =
+// x[2r] + i x[2r+1] are transformed along r as a column four-step.  Pass A
+// leaves Y[k1][j][c]; the L-point FFT over j gives Z[k1 + h1*k2][c] at bin
+// k2.  The real pass over row pairs (multi.hpp:343-414, forward) forms
+//     X[k] = (Z[k] + conj Z[h-k]) / 2 - i W_2h^k (Z[k] - conj Z[h-k]) / 2,
+//     X[h-k] = conj of the two halves' terms in the same way,
+// per column, and h - k = (h1 - k1) + h1*(L - 1 - k2) for k1 > 0, h1*((L -
+// k2) mod L) for k1 = 0.  A work-group takes a block of CT columns and the
+// tiles k1 = q and h1 - q (q = 0 .. h1/2; q = 0 and q = h1/2 pair with
+// themselves): both column FFTs, tile h1 - q's bins through the LDS, then
+// X for each bin k2 of tile q and its partner bin, written straight from the
+// registers (rows of CT consecutive columns).  This replaces pass B plus the
+// separate real pass, which read and wrote the output rows once more.
+template <typename T, int L>
+constexpr int rcp_ct() {
+  // 34 KiB of LDS per group (one tile of L-point columns)
+  return (sizeof(T) == 4 ? 64 : 32) * 64 / L;
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__((PassShape<L, rcp_ct<T, L>(), 0>::NT)) real_colpair_kernel(RealColPairArgs<T> a) {
+  constexpr int CT = rcp_ct<T, L>();
+  using PS = PassShape<L, CT, 0>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  __shared__ cx<T> lds[CT * LdsLen<L>::value];
+
+  const int tid = threadIdx.x;
+  const int c = tid % CT, t = tid / CT;
+  const idx_t ncb = a.m / CT;
+  const int nq = a.h1 / 2 + 1;
+  idx_t g = blockIdx.x;
+  const idx_t cb = g % ncb;
+  g /= ncb;
+  const int q = (int)(g % nq);
+  const idx_t b = g / nq;
+  const int ka = q;
+  const int kb = (a.h1 - q) & (a.h1 - 1);
+  const idx_t col = cb * CT + c;
+  const cx<T>* __restrict__ src = a.in + b * a.in_bs + col;
+
+  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
+  cx<T> va[E], vb[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    va[mm] = src[((idx_t)ka * L + t + mm * TPT) * a.m];
+  });
+  if (kb != ka) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int mm = decltype(mc)::value;
+      vb[mm] = src[((idx_t)kb * L + t + mm * TPT) * a.m];
+    });
+  }
+  run_stages<T, L, 0, 0, false, CT>(va, t, lat, a.stage_tw);
+  // (kb != ka is uniform over the work-group: the barriers inside match)
+  if (kb != ka) {
+    run_stages<T, L, 0, 0, false, CT>(vb, t, lat, a.stage_tw);
+  } else {
+    static_for<E>([&](auto mc) AFGPU_INLINE { vb[decltype(mc)::value] = va[decltype(mc)::value]; });
+  }
+  // every exchange of run_stages ends with a barrier after its reads, so the
+  // LDS is free: tile kb's bins k2 = t + mm*TPT at entry k2*CT + c
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    lds[(t + mm * TPT) * CT + c] = vb[mm];
+  });
+  __syncthreads();
+
+  const idx_t h = (idx_t)a.h1 * L;
+  const idx_t ob = b * a.out_bs + col;
+  auto st = [&](idx_t row, cx<T> v) AFGPU_INLINE {
+    const idx_t e = ob + row * a.m;
+    if (a.out_il) {
+      reinterpret_cast<cx<T>*>(a.out_re)[e] = v;
+    } else {
+      a.out_re[e] = v.x;
+      a.out_im[e] = v.y;
+    }
+  };
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    const int k2 = t + mm * TPT;
+    // a tile paired with itself writes each pair once
+    const bool mine = ka != kb || (ka == 0 ? k2 <= L / 2 : k2 < L / 2);
+    const int pk2 = ka == 0 ? ((L - k2) & (L - 1)) : (L - 1 - k2);
+    const cx<T> s1 = lds[pk2 * CT + c];
+    if (mine) {
+      const idx_t k = (idx_t)ka + (idx_t)a.h1 * k2;
+      const cx<T> s0 = va[mm];
+      const cx<T> w = a.real_tw[k];
+      const cx<T> s1c = conjg(s1);
+      const T hf = T(0.5);
+      const cx<T> aa = {(s0.x + s1c.x) * hf, (s0.y + s1c.y) * hf};
+      const cx<T> bw = cmul(s0 - s1c, w);
+      const cx<T> bb = {bw.x * hf, bw.y * hf};
+      st(k, aa + mul_neg_i(bb));
+      if (h - k != k) st(h - k, conjg(aa) + mul_neg_i(conjg(bb)));
+    }
+  });
+}
+
+int real_colpair_ct(int elem_bytes, int L, idx_t m, int h1) {
+  int ct = 0;
+  if (L == 64) ct = elem_bytes == 4 ? rcp_ct<float, 64>() : rcp_ct<double, 64>();
+  if (L == 128) ct = elem_bytes == 4 ? rcp_ct<float, 128>() : rcp_ct<double, 128>();
+  if (L == 256) ct = elem_bytes == 4 ? rcp_ct<float, 256>() : rcp_ct<double, 256>();
+  if (ct == 0 || m < ct || m % ct != 0 || h1 < 2 || (h1 & (h1 - 1)) != 0) return 0;
+  return ct;
+}
+
+template <typename T, int L>
+static hipError_t launch_rcp(const RealColPairArgs<T>& a, hipStream_t st) {
+  constexpr int CT = rcp_ct<T, L>();
+  const idx_t blocks = a.nbatch * (idx_t)(a.h1 / 2 + 1) * (a.m / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > INT_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((real_colpair_kernel<T, L>), dim3((unsigned)blocks), dim3(PassShape<L, CT, 0>::NT), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t real_colpair(const RealColPairArgs<T>& a, hipStream_t st) {
+  if (real_colpair_ct((int)sizeof(T), a.L, a.m, a.h1) == 0) return hipErrorInvalidValue;
+  switch (a.L) {
+    case 64: return launch_rcp<T, 64>(a, st);
+    case 128: return launch_rcp<T, 128>(a, st);
+    default: return launch_rcp<T, 256>(a, st);
+  }
+}
+
+template hipError_t real_colpair<float>(const RealColPairArgs<float>&, hipStream_t);
+template hipError_t real_colpair<double>(const RealColPairArgs<double>&, hipStream_t);
 template hipError_t real_irowfft<float>(const RealIRowFftArgs<float>&, hipStream_t);
 template hipError_t real_irowfft<double>(const RealIRowFftArgs<double>&, hipStream_t);
 template hipError_t real_rowfft<float>(const RealRowFftArgs<float>&, hipStream_t);

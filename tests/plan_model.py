@@ -255,6 +255,36 @@ def _real_rows(p, bufs, start, cnt, pi):
         _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
 
 
+def _real_colpair(p, bufs, start, cnt, pi):
+    """Column pass B of the dim-0 four-step with the forward real pass over
+    row pairs (afft_realpair.hip real_colpair_kernel): Y[k1][j][c] (h1 x L x
+    m) -> Z[k1 + h1*k2][c], the L-point DFT over j; then rows 0..h of X as
+    in _real_rows."""
+    h, m, h1, L = p.rr_h, p.rr_m, p.n_blo, p.L
+    assert h == h1 * L
+    b = np.arange(cnt).reshape(cnt, 1, 1, 1)
+    k1 = np.arange(h1).reshape(1, h1, 1, 1)
+    j = np.arange(L).reshape(1, 1, L, 1)
+    c = np.arange(m).reshape(1, 1, 1, m)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    e_in = uin + b * p.in_.s_b + k1 * p.in_.s_b2 + j * p.in_.s_j + c * (p.in_.s_c or 1)
+    Y = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    Z = np.fft.fft(Y, axis=2).transpose(0, 2, 1, 3).reshape(cnt, h, m)  # row k1 + h1*k2
+    k = np.arange(h).reshape(1, h, 1)
+    Zh = Z[:, (-np.arange(h)) % h, :]
+    w = np.exp(-2j * np.pi * k / (2 * h))
+    X = 0.5 * (Z + np.conj(Zh)) - 1j * 0.5 * (Z - np.conj(Zh)) * w
+    X[:, 0, :] = Z[:, 0, :].real + Z[:, 0, :].imag
+    last = (Z[:, 0, :].real - Z[:, 0, :].imag)[:, None, :]
+    X = np.concatenate([X, last], axis=1)
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    rows = np.arange(h + 1).reshape(1, h + 1, 1)
+    bo = np.arange(cnt).reshape(cnt, 1, 1)
+    co = np.arange(m).reshape(1, 1, m)
+    e_out = uout + bo * p.out.s_b + rows * p.out.s_j + co * (p.out.s_c or 1)
+    _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
+
+
 def _real_tr(p, bufs, start, cnt, pi):
     """Transposing forward real pass (afft_aux.hip real_tr_kernel): input
     Y[b][k1][k2] (n1 x n2) holds Z[k1 + n1*k2]; output bins 0..h natural."""
@@ -421,6 +451,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant
                 _real_rowfft(p, bufs, start, cnt, pi)
             elif p.type == 6:
                 _real_irowfft(p, bufs, start, cnt, pi)
+            elif p.type == 7:
+                _real_colpair(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

@@ -140,12 +140,33 @@ def test_plan_model_matches_oracle(kind, dims):
 @pytest.mark.parametrize("kind", ["r", "ri"])
 def test_plan_model_real_pass_in_rows(kind):
     # long real dim 0 with a 1024-point last axis: the real pass over row
-    # pairs runs inside the row FFTs (pass type 5, or 6 for c2r)
+    # pairs runs inside the row FFTs (pass type 6 for c2r; for r2c type 5 on
+    # request, by default type 7, inside the column pass B)
     for fmt in (IL, SP):
         for batch in (1, 3):
             err, info, passes = model_check(kind, (8192, 1024), fmt, batch)
-            assert [p.type for p in passes] == ([0, 0, 5] if kind == "r" else [6, 0, 0])
+            assert [p.type for p in passes] == ([0, 7, 0] if kind == "r" else [6, 0, 0])
             assert err < 1e-9, (kind, fmt, batch, err)
+            if kind == "r":
+                err, info, passes = model_check(kind, (8192, 1024), fmt, batch, variant={"real_pair": 1})
+                assert [p.type for p in passes] == [0, 0, 5]
+                assert err < 1e-9, (kind, fmt, batch, err)
+
+
+@pytest.mark.parametrize("dims", [(8192, 64), (16384, 64), (65536, 16), (8192, 2, 64)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_plan_model_real_pass_in_column_pass_b(dims):
+    # long real dim 0 whose rows take no fused row pass: the real pass over
+    # row pairs runs in pass B of the dim-0 column four-step (64-, 128- and
+    # 256-point columns), and with real_pair = -1 as a pass of its own
+    for fmt in (IL, SP):
+        for v in (None, {"real_pair": -1}):
+            err, info, passes = model_check("r", dims, fmt, 2, variant=v)
+            assert (7 in [p.type for p in passes]) == (v is None), [(p.type, p.L) for p in passes]
+            assert err < 1e-9, (dims, fmt, v, err)
+    # and in FP64 (half the columns per tile)
+    err, info, passes = model_check("r", dims, IL, 1, prec=64)
+    assert 7 in [p.type for p in passes] and err < 1e-9
 
 
 @pytest.mark.parametrize("lg", [16, 20, 21, 22])
@@ -208,18 +229,28 @@ def test_plan_shapes_of_headline_configs():
     # 2^12 c2c: one row pass
     info, passes = M.inspect("c", (4096,), IL, 1)
     assert [(p.L, p.colmap) for p in passes] == [(4096, 0)]
-    # 8192^2 r2c: dim 0 (4096 complex rows) as a 64 x 64 column four-step,
-    # the real pass over row pairs, then the 8192-point rows
+    # 8192^2 r2c: dim 0 (4096 complex rows) as a 64 x 64 column four-step
+    # whose pass B runs the real pass over row pairs, then the 8192-point rows
     info, passes = M.inspect("r", (8192, 8192), IL, 1)
+    assert [(p.type, p.L, p.colmap, p.epi) for p in passes] == [(0, 64, 1, 1), (7, 64, 1, 0), (0, 8192, 0, 0)]
+    assert passes[0].tw4_blo == 1 and passes[1].rr_h == 4096 and passes[1].rr_m == 8192 and passes[1].n_blo == 64
+    # the same with the real pass on its own
+    info, passes = M.inspect("r", (8192, 8192), IL, 1, variant={"real_pair": -1})
     assert [(p.type, p.L, p.colmap, p.epi) for p in passes] == [(0, 64, 1, 1), (0, 64, 1, 0), (1, 1, 0, 0),
                                                                  (0, 8192, 0, 0)]
-    assert passes[0].tw4_blo == 1 and passes[2].rr_h == 4096 and passes[2].rr_m == 8192
-    # 16384 x 2048 r2c: the real pass inside the 2048-point row FFTs
+    assert passes[2].rr_h == 4096 and passes[2].rr_m == 8192
+    # 16384 x 2048 r2c: the real pass inside the 128-point column pass B, or
+    # on request inside the 2048-point row FFTs
     info, passes = M.inspect("r", (16384, 2048), IL, 1)
+    assert [(p.type, p.L) for p in passes] == [(0, 64), (7, 128), (0, 2048)] and passes[1].rr_h == 8192
+    info, passes = M.inspect("r", (16384, 2048), IL, 1, variant={"real_pair": 1})
     assert [(p.type, p.L) for p in passes][-1] == (5, 2048) and passes[-1].rr_h == 8192
-    # 8192 x 64 r2c (no fused row kernel for 64-point rows): the real pass of its own
+    # 8192 x 64 r2c (no fused row kernel for 64-point rows): the real pass in
+    # the column pass B; 8192 x 8 (fewer columns than its tile): on its own
     info, passes = M.inspect("r", (8192, 64), IL, 1)
-    assert [p.type for p in passes] == [0, 0, 1, 0] and passes[2].rr_h == 4096 and passes[2].rr_m == 64
+    assert [p.type for p in passes] == [0, 7, 0] and passes[1].rr_h == 4096 and passes[1].rr_m == 64
+    info, passes = M.inspect("r", (8192, 8), IL, 1)
+    assert [p.type for p in passes] == [0, 0, 1, 0] and passes[2].rr_h == 4096 and passes[2].rr_m == 8
     # 2048 x 8192 r2c: dim 0 short enough for one fused column r2c pass
     info, passes = M.inspect("r", (2048, 8192), IL, 1)
     assert [(p.L, p.colmap, p.epi) for p in passes] == [(1024, 1, 2), (8192, 0, 0)]

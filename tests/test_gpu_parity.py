@@ -89,7 +89,11 @@ ND_SHAPES = [(2, 4), (4, 1, 8), (16, 32), (32, 16), (8, 8, 8), (2, 2, 2, 2), (81
              (2, 8192), (32, 16384),
              # long strided axes and long real dim 0 (column four-step, real pass over row pairs)
              (16384, 8), (4096, 4, 4), (8192, 64),
-             # long real dim 0 with 1024..8192-point rows: the real pass inside the row FFTs
+             # pass B of the dim-0 column four-step with the real pass (64-, 128-
+             # and 256-point columns)
+             (16384, 64), (65536, 16),
+             # long real dim 0 with 1024..8192-point rows: the real pass inside the
+             # column pass B (r2c) or the inverse row FFTs (c2r)
              (8192, 1024), (8192, 2048)]
 
 
@@ -111,7 +115,47 @@ def test_nd(G, shape):
         assert O.rel_l2(G.gpu_c2r(h, shape, fmt).astype(np.float32), refri) <= O.bound(n)
 
 
-@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,)],
+@pytest.mark.parametrize("shape", [(8192, 64), (8192, 1024), (16384, 2048), (8192, 4, 64)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_nd_r2c_real_pass_forms(G, shape):
+    """N-D r2c with a long dim 0: the real pass over row pairs inside the
+    column pass B (default), inside the row FFTs (real_pair = 1, 2-D with
+    1024..8192-point rows) and on its own (-1), each against the oracle, in
+    both formats and a batch of 2."""
+    import torch
+    from fft_amd.afft import Plan
+    n = int(np.prod(shape))
+    m = int(np.prod(G.half(shape)))
+    xr = O.normal(9, 2 * n).reshape((2,) + shape)
+    ref = [O.rfft(xr[b].astype(np.float64)).ravel() for b in range(2)]
+    src = G.to_dev(xr.ravel())
+    seen = set()
+    for fmt in (G.IL, G.SP):
+        p = Plan("r", shape, 2, format=fmt)
+        for rp in (0, 1, -1):
+            p.set_variant(real_pair=rp)
+            desc = tuple(p.describe())
+            seen.add(desc)
+            say("nd r2c real pass", shape, fmt, rp, desc)
+            if fmt == G.IL:
+                dst = G.empty_dev(4 * m)
+                p(src, None, dst, None)
+                torch.cuda.synchronize()
+                y = G.to_host(dst)
+                y = (y[0::2] + 1j * y[1::2]).reshape(2, m)
+            else:
+                dre, dim_ = G.empty_dev(2 * m), G.empty_dev(2 * m)
+                p(src, None, dre, dim_)
+                torch.cuda.synchronize()
+                y = (G.to_host(dre) + 1j * G.to_host(dim_)).reshape(2, m)
+            for b in range(2):
+                assert O.rel_l2(y[b].astype(np.complex64), ref[b]) <= O.bound(n), (shape, fmt, rp, b)
+        p.close()
+    assert any("real_colpair" in d for s_ in seen for d in s_), seen
+    assert any("real_rows" in d for s_ in seen for d in s_), seen
+
+
+@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fp64_path(G, shape):
     n = int(np.prod(shape))
@@ -211,14 +255,16 @@ def test_large_nd_against_afft_summaries(G):
         Plan(kind, shape, dtype=np.float64)(src.double(), None, dst64, None)
         if kind == "r":
             # the opt-in fused real passes (real pass inside the 8192-point
-            # rows; 1-D pass B with the real post-pass) at full size
-            pf = Plan(kind, shape)
-            pf.set_variant(real_pair=1)
-            pf(src, None, dst, None)
-            torch.cuda.synchronize()
-            say("  fused", pf.describe()[-1])
-            assert G.device_rel_l2(dst, dst64) <= O.bound(n)
-            pf.close()
+            # rows; 1-D pass B with the real post-pass) and the real pass on
+            # its own, at full size
+            for rp in (1, -1):
+                pf = Plan(kind, shape)
+                pf.set_variant(real_pair=rp)
+                pf(src, None, dst, None)
+                torch.cuda.synchronize()
+                say("  real_pair", rp, pf.describe())
+                assert G.device_rel_l2(dst, dst64) <= O.bound(n)
+                pf.close()
         Plan(kind, shape)(src, None, dst, None)
         torch.cuda.synchronize()
         assert G.device_rel_l2(dst, dst64) <= O.bound(n)
