@@ -117,8 +117,30 @@ struct RealColPairArgs {
   const cx<T>* real_tw;   // W_{2h}^k, k < h
 };
 
-// Columns per work-group of real_colpair (0: no kernel for these shapes).
+// Columns per work-group of real_colpair / real_icolpair (0: no kernel for
+// these shapes; L = the column FFT length, h1 = the other factor of h).
 int real_colpair_ct(int elem_bytes, int L, idx_t m, int h1);
+
+// N-D c2r with the halved dim 0 as a column four-step h = L * h2: the inverse
+// real pass over row pairs (k, h-k) fused into pass A (the L-point FFTs over
+// j of the rows j*h2 + n2 with the W_h^(n2*k1) twiddle; afft_realpair.hip).
+// Input: spectrum rows 0..h of m entries (row r at r*m, interleaved); output:
+// the plan's scratch, entry (k1, n2, c) at k1*h2*m + n2*m + c.
+template <typename T>
+struct RealIColPairArgs {
+  const cx<T>* in;
+  cx<T>* out;
+  int L;                  // points of the column FFTs (h1)
+  int h2;                 // column blocks n2
+  idx_t m;                // columns
+  idx_t in_bs, out_bs;    // batch strides (complex entries)
+  idx_t nbatch;
+  const cx<T>* stage_tw;  // stage twiddles of the L-point columns (radix-16 schedule)
+  const cx<T>* real_tw;   // W_{2h}^k, k < h
+  const cx<T>* tw4_hi;    // W_h^e = tw4_hi[e >> tw4_bits] * tw4_lo[e & mask]
+  const cx<T>* tw4_lo;
+  int tw4_bits;
+};
 
 // all = false: only the row lengths where the fused pass is the default
 bool real_rowfft_supported(int elem_bytes, idx_t h, int L, bool all);
@@ -144,6 +166,8 @@ template <typename T>
 hipError_t real_irowfft(const RealIRowFftArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_colpair(const RealColPairArgs<T>& a, hipStream_t st);
+template <typename T>
+hipError_t real_icolpair(const RealIColPairArgs<T>& a, hipStream_t st);
 template <typename T>
 hipError_t real_one(const T* src, T* dst_re, T* dst_im, int inverse, int il, idx_t nb, idx_t in_s, idx_t out_s,
                     hipStream_t st);

@@ -77,7 +77,8 @@ enum PassType {
   PT_REALPAIR = 4,
   PT_REALROWFFT = 5,
   PT_REALIROWFFT = 6,
-  PT_REALCOLPAIR = 7
+  PT_REALCOLPAIR = 7,
+  PT_REALICOLPAIR = 8
 };
 
 struct Side {
@@ -718,7 +719,45 @@ struct Builder {
   void real_dim0_fourstep_c2r(const Side& cur, const Side& usr_dst, idx_t n0, idx_t inner, bool real_pass = true) {
     const idx_t h = n0 / 2;
     const idx_t h1 = (idx_t)1 << (ilog2(h) / 2), h2 = h / h1;
-    if (real_pass) {
+    const int rct = real_pass && P.var.real_pair >= 0 && cur.il && cur.role >= R_SCR
+                        ? real_colpair_ct((int)sizeof(T), (int)h1, inner, (int)h2)
+                        : 0;
+    if (rct > 0) {
+      // the inverse real pass over row pairs inside pass A: blocks n2 and
+      // h2 - n2 per work-group (afft_realpair.hip)
+      need_scratch(R_SCR2, h * inner);
+      const int bits = (ilog2(h) + 1) / 2;
+      const void* tw4 = get_table<T>(1, h, bits);
+      PassRec* r = add();
+      if (!r || !tw4) {
+        ok = false;
+        return;
+      }
+      r->type = PT_REALICOLPAIR;
+      r->L = (int)h1;
+      r->CT = rct;
+      r->colmap = 1;
+      r->in = cur;
+      r->in.s_j = inner;
+      r->in.s_c = 1;
+      r->in.s_b = (h + 1) * inner;
+      scratch_side(r->out, R_SCR2);
+      r->out.s_j = h2 * inner;
+      r->out.s_c = 1;
+      r->out.s_b = h * inner;
+      r->n_col = inner;
+      r->n_blo = (int)h2;
+      r->rr_h = h;
+      r->rr_m = inner;
+      r->rr_inv = 1;
+      r->stage_tw = get_table<T>(0, (int)h1, 0);
+      r->real_tw = get_table<T>(2, h, 0);
+      r->tw4_hi = tw4;
+      r->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
+      r->tw4_bits = bits;
+      if (!r->stage_tw || !r->real_tw) ok = false;
+    }
+    if (real_pass && rct == 0) {
       Side rr = cur;
       rr.s_j = inner;
       rr.s_c = 1;
@@ -733,23 +772,25 @@ struct Builder {
       return;
     }
     // pass A: h1-point columns of rows n1*h2 + n2 (column index n2*inner + c)
-    PassRec* a = add();
-    if (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0)) return;
-    a->in = cur;
-    a->in.s_j = h2 * inner;
-    a->in.s_c = 1;
-    a->in.s_b = (h + 1) * inner;
-    scratch_side(a->out, R_SCR2);
-    a->out.s_j = h2 * inner;
-    a->out.s_c = 1;
-    a->out.s_b = h * inner;
-    a->n_col = h2 * inner;
-    a->b_per_user = 1;
-    a->swap_in = 1;
-    a->tw4_hi = tw4;
-    a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
-    a->tw4_bits = bits;
-    a->tw4_shift = ilog2(inner);
+    PassRec* a = rct > 0 ? nullptr : add();
+    if (rct == 0 && (!a || !fft_pass(a, (int)h1, 1, EPI_TW4, 0))) return;
+    if (a) {
+      a->in = cur;
+      a->in.s_j = h2 * inner;
+      a->in.s_c = 1;
+      a->in.s_b = (h + 1) * inner;
+      scratch_side(a->out, R_SCR2);
+      a->out.s_j = h2 * inner;
+      a->out.s_c = 1;
+      a->out.s_b = h * inner;
+      a->n_col = h2 * inner;
+      a->b_per_user = 1;
+      a->swap_in = 1;
+      a->tw4_hi = tw4;
+      a->tw4_lo = (const cx<T>*)tw4 + (h >> bits);
+      a->tw4_bits = bits;
+      a->tw4_shift = ilog2(inner);
+    }
     // pass B: h2-point columns (batch index k1), real rows out
     PassRec* b = add();
     if (!b || !fft_pass(b, (int)h2, 1, EPI_PLAIN, 0, 4 * col_tile<T>((int)h2))) return;
@@ -897,8 +938,10 @@ struct Builder {
         // four-step (real_dim0_fourstep_r2c; 8192 x 1024: 0.048 ms, 16384 x
         // 4096: 0.337 ms) or, on request (real_pair = 1, 2-D with a compiled
         // row length), inside the row FFTs (0.056 and 0.375 ms)
-        const bool fuse =
-            nd == 2 && P.var.real_pair > 0 && real_rowfft_supported((int)sizeof(T), h, (int)inner, true);
+        const idx_t f1 = (idx_t)1 << (ilog2(h) / 2);
+        const bool col_ok = real_colpair_ct((int)sizeof(T), (int)(h / f1), inner, (int)f1) > 0;
+        const bool fuse = nd == 2 && P.var.real_pair >= 0 && (P.var.real_pair > 0 || !col_ok) &&
+                          real_rowfft_supported((int)sizeof(T), h, (int)inner, P.var.real_pair > 0);
         real_dim0_fourstep_r2c(usr_src, usr_dst, n0, inner, !fuse);
         if (fuse) {
           PassRec* r = add();
@@ -1017,9 +1060,14 @@ struct Builder {
     scratch_side(scr, R_SCR);
     need_scratch(R_SCR, (h + 1) * inner);
     Side cur = in;
-    // 2-D with a long dim 0 and a compiled row length: the inverse real pass
-    // over row pairs runs inside the row FFTs (afft_realpair.hip)
-    if (h > kColDirectMax && nd == 2 && P.var.real_pair >= 0 &&
+    // long dim 0: the inverse real pass over row pairs runs inside pass A of
+    // the dim-0 column four-step (real_dim0_fourstep_c2r; 16384 x 2048: 0.167
+    // ms, 16384 x 4096: 0.371 ms) or, on request (real_pair = 1, 2-D with a
+    // compiled row length) or where that has no kernel, inside the inverse row
+    // FFTs (0.213 and 0.418 ms; 8192 x 1024: 0.053 against 0.055 ms)
+    const idx_t f1 = h > 1 ? (idx_t)1 << (ilog2(h) / 2) : 1;
+    const bool col_ok = real_colpair_ct((int)sizeof(T), (int)f1, inner, (int)(h / f1)) > 0;
+    if (h > kColDirectMax && nd == 2 && P.var.real_pair >= 0 && (P.var.real_pair > 0 || !col_ok) &&
         real_rowfft_supported((int)sizeof(T), h, (int)inner, P.var.real_pair > 0)) {
       PassRec* r = add();
       if (!r) return false;
@@ -1315,6 +1363,24 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.real_tw = (const cx<T>*)r.real_tw;
     a.row_tw = (const cx<T>*)r.tw4_hi;
     HIP_OK(real_pair<T>(a, st));
+    return true;
+  }
+  if (r.type == PT_REALICOLPAIR) {
+    RealIColPairArgs<T> a{};
+    a.in = reinterpret_cast<const cx<T>*>(ire);
+    a.out = reinterpret_cast<cx<T>*>(ore);
+    a.L = r.L;
+    a.h2 = r.n_blo;
+    a.m = r.rr_m;
+    a.in_bs = r.in.s_b;
+    a.out_bs = r.out.s_b;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    a.tw4_hi = (const cx<T>*)r.tw4_hi;
+    a.tw4_lo = (const cx<T>*)r.tw4_lo;
+    a.tw4_bits = r.tw4_bits;
+    HIP_OK(real_icolpair<T>(a, st));
     return true;
   }
   if (r.type == PT_REALCOLPAIR) {
@@ -1734,7 +1800,8 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   // plans with a real pass that has a fused form also try the other form
   for (int i = 0; i < P.npass; i++)
     if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT ||
-        P.pass[i].type == PT_REALIROWFFT || P.pass[i].type == PT_REALCOLPAIR) {
+        P.pass[i].type == PT_REALIROWFFT || P.pass[i].type == PT_REALCOLPAIR ||
+        P.pass[i].type == PT_REALICOLPAIR) {
       Variant v;
       v.real_pair = P.pass[i].type == PT_REALTR ? 1 : -1;
       cands.push_back(v);
@@ -2070,6 +2137,9 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
              (long long)r.rr_m, r.out.role, r.out.il);
   else if (r.type == PT_REALROWS)
     snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
+  else if (r.type == PT_REALICOLPAIR)
+    snprintf(buf, len, "real_icolpair L=%d CT=%d h2=%d h=%lld m=%lld in(role=%d il=%d)", r.L, r.CT, r.n_blo,
+             (long long)r.rr_h, (long long)r.rr_m, r.in.role, r.in.il);
   else if (r.type == PT_REALCOLPAIR)
     snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
              (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);

@@ -596,6 +596,146 @@ hipError_t real_colpair(const RealColPairArgs<T>& a, hipStream_t st) {
   }
 }
 
+// ---- column pass A of a dim-0 four-step with the inverse real pass --------
+//
+// N-D c2r over a dim 0 of n0 = 2h rows (h = L * h2): after the inner axes the
+// spectrum rows S[0..h] are paired by the inverse real pass (multi.hpp:343-414,
+// inverse)
+//     Z[k] = (S[k] + conj S[h-k]) - i conj(W_2h^k) (conj S[h-k] - S[k]),
+//     Z[0] = (Re S[0] + Re S[h]) + i (Re S[0] - Re S[h]),
+// and the inverse h-point c2c of Z runs as a column four-step whose pass A
+// transforms, for each n2, the L-point column over j of rows k = j*h2 + n2.
+// h - k = (L - 1 - j)*h2 + (h2 - n2) for n2 > 0 and (L - j)*h2 for n2 = 0, so
+// the partners of block n2 lie in block h2 - n2.  A work-group takes a block
+// of CT columns of blocks n2 = q and h2 - q (q = 0 .. h2/2; q = 0 and h2/2
+// pair with themselves, block 0 also with row h), forms Z for both from the
+// raw rows (each tile's partner values cross the LDS), and runs pass A on
+// both: re / im swapped in (the inverse as a forward transform), the column
+// FFTs, the W_h^(n2*k1) twiddle, the store into the scratch.  This replaces
+// the separate inverse real pass, which read and wrote the spectrum once more.
+template <typename T, int L>
+__global__ void __launch_bounds__((PassShape<L, rcp_ct<T, L>(), 0>::NT)) real_icolpair_kernel(RealIColPairArgs<T> a) {
+  constexpr int CT = rcp_ct<T, L>();
+  using PS = PassShape<L, CT, 0>;
+  constexpr int E = PS::E;
+  constexpr int TPT = PS::TPT;
+  __shared__ cx<T> lds[CT * LdsLen<L>::value];
+
+  const int tid = threadIdx.x;
+  const int c = tid % CT, t = tid / CT;
+  const idx_t ncb = a.m / CT;
+  const int nq = a.h2 / 2 + 1;
+  idx_t g = blockIdx.x;
+  const idx_t cb = g % ncb;
+  g /= ncb;
+  const int q = (int)(g % nq);
+  const idx_t b = g / nq;
+  const int na = q;
+  const int nb = (a.h2 - q) & (a.h2 - 1);
+  const bool self = na == nb;
+  const idx_t col = cb * CT + c;
+  const idx_t h = (idx_t)L * a.h2;
+  const cx<T>* __restrict__ src = a.in + b * a.in_bs + col;
+
+  // raw rows j*h2 + n2 of both blocks, j = t + mm*TPT
+  cx<T> va[E], vb[E], pa[E], pb[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    va[mm] = src[((idx_t)(t + mm * TPT) * a.h2 + na) * a.m];
+  });
+  if (!self) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int mm = decltype(mc)::value;
+      vb[mm] = src[((idx_t)(t + mm * TPT) * a.h2 + nb) * a.m];
+    });
+  } else {
+    static_for<E>([&](auto mc) AFGPU_INLINE { vb[decltype(mc)::value] = va[decltype(mc)::value]; });
+  }
+  // row h: the partner of row 0 (block 0, j = 0: thread t = 0, register 0)
+  const cx<T> sh = (na == 0 && t == 0) ? src[h * a.m] : cx<T>{T(0), T(0)};
+  // partner entry j' of entry j of a block n2: L - 1 - j, or (L - j) mod L for n2 = 0
+  auto pj = [&](int n2, int j) AFGPU_INLINE -> int { return n2 == 0 ? ((L - j) & (L - 1)) : (L - 1 - j); };
+  // partners of block na's entries from block nb's raw values, then the reverse
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    lds[(t + mm * TPT) * CT + c] = vb[mm];
+  });
+  __syncthreads();
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    pa[mm] = lds[pj(na, t + mm * TPT) * CT + c];
+  });
+  __syncthreads();
+  if (!self) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int mm = decltype(mc)::value;
+      lds[(t + mm * TPT) * CT + c] = va[mm];
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int mm = decltype(mc)::value;
+      pb[mm] = lds[pj(nb, t + mm * TPT) * CT + c];
+    });
+    __syncthreads();
+  }
+  // Z of every entry, re / im swapped for the forward-transform inverse
+  auto pre = [&](cx<T> s0, cx<T> s1, idx_t k) AFGPU_INLINE -> cx<T> {
+    const cx<T> s1c = conjg(s1);
+    const cx<T> aa = s0 + s1c;
+    const cx<T> bb = cmulc(s1c - s0, a.real_tw[k]);
+    return swapri(aa + mul_neg_i(bb));
+  };
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int mm = decltype(mc)::value;
+    const int j = t + mm * TPT;
+    const idx_t ka = (idx_t)j * a.h2 + na;
+    if (mm == 0 && na == 0 && t == 0)
+      va[mm] = swapri(cx<T>{va[mm].x + sh.x, va[mm].x - sh.x});
+    else
+      va[mm] = pre(va[mm], pa[mm], ka);
+    if (!self) vb[mm] = pre(vb[mm], pb[mm], (idx_t)j * a.h2 + nb);
+  });
+
+  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
+  const unsigned mask = (1u << a.tw4_bits) - 1u;
+  cx<T>* __restrict__ dst = a.out + b * a.out_bs + col;
+  // column FFTs, twiddle W_h^(n2*k1), bins k1 = t + mm*TPT to rows k1*h2 + n2
+  auto finish = [&](cx<T>(&v)[E], int n2) AFGPU_INLINE {
+    run_stages<T, L, 0, 0, false, CT>(v, t, lat, a.stage_tw);
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int mm = decltype(mc)::value;
+      const int k1 = t + mm * TPT;
+      const cx<T> w = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)n2 * (unsigned)k1);
+      dst[((idx_t)k1 * a.h2 + n2) * a.m] = cmul(v[mm], w);
+    });
+  };
+  finish(va, na);
+  // (self is uniform over the work-group: the barriers inside match)
+  if (!self) finish(vb, nb);
+}
+
+template <typename T, int L>
+static hipError_t launch_ircp(const RealIColPairArgs<T>& a, hipStream_t st) {
+  constexpr int CT = rcp_ct<T, L>();
+  const idx_t blocks = a.nbatch * (idx_t)(a.h2 / 2 + 1) * (a.m / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > INT_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((real_icolpair_kernel<T, L>), dim3((unsigned)blocks), dim3(PassShape<L, CT, 0>::NT), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t real_icolpair(const RealIColPairArgs<T>& a, hipStream_t st) {
+  if (real_colpair_ct((int)sizeof(T), a.L, a.m, a.h2) == 0) return hipErrorInvalidValue;
+  switch (a.L) {
+    case 64: return launch_ircp<T, 64>(a, st);
+    case 128: return launch_ircp<T, 128>(a, st);
+    default: return launch_ircp<T, 256>(a, st);
+  }
+}
+
+template hipError_t real_icolpair<float>(const RealIColPairArgs<float>&, hipStream_t);
+template hipError_t real_icolpair<double>(const RealIColPairArgs<double>&, hipStream_t);
 template hipError_t real_colpair<float>(const RealColPairArgs<float>&, hipStream_t);
 template hipError_t real_colpair<double>(const RealColPairArgs<double>&, hipStream_t);
 template hipError_t real_irowfft<float>(const RealIRowFftArgs<float>&, hipStream_t);

@@ -106,15 +106,14 @@ typedef struct
   int32_t pipeline;  /* two-pass four-step plans over several groups: 1 runs pass A of
                         group k and pass B of group k-1 in one fused launch (double-
                         buffered scratch); 0 (default) or -1: one pass per launch */
-  int32_t real_pair; /* fused real passes.  N-D r2c with dim0 > 4096: the real
-                        pass over row pairs (k, h-k) runs inside pass B of the
-                        dim-0 column four-step (0, default; 64..256-point
-                        columns), with 1 and a 1024..8192-point last axis of a
-                        2-D transform inside the row FFTs, with -1 as a pass of
-                        its own.  2-D c2r with dim0 > 4096 and a 1024..8192-point
-                        last axis: the inverse real pass runs inside the row
-                        FFTs (0: rows up to 4096 points, 1: all, -1: a pass of
-                        its own).  FP32 1-D c2r with
+  int32_t real_pair; /* fused real passes.  N-D r2c / c2r with dim0 > 4096: the
+                        (inverse) real pass over row pairs (k, h-k) runs inside
+                        pass B (r2c) or pass A (c2r) of the dim-0 column
+                        four-step (0, default; 64..256-point columns); with 1
+                        (or, where that has no kernel, rows of up to 4096
+                        points) and a 1024..8192-point last axis of a 2-D
+                        transform, inside the row FFTs; with -1 as a pass of
+                        its own.  FP32 1-D c2r with
                         2^20 <= N <= 2^22: 1 runs the inverse real pre-pass in the
                         load of four-step pass A (measured slower); 0 or -1: a
                         pass of its own.  FP32 1-D r2c,
@@ -227,7 +226,11 @@ typedef struct
                        with the real pass over row pairs fused in (input
                        entry (k1, j, c) at k1*in.s_b2 + j*in.s_j + c, output
                        rows 0..h at out.s_j; n_blo = h1, rr_h = h, rr_m = the
-                       number of columns) */
+                       number of columns),
+                   8 = N-D c2r: the inverse real pass over row pairs fused
+                       into pass A of the dim-0 column four-step h = L * h2
+                       (input rows 0..h at in.s_j, output entry (k1, n2, c) at
+                       k1*out.s_j + n2*rr_m + c; n_blo = h2, rr_h = h) */
   int32_t L, CT, colmap, epi, store_t;
   afft_gpu_side_info in, out;
   int64_t n_col;

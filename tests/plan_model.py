@@ -285,6 +285,37 @@ def _real_colpair(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
 
 
+def _real_icolpair(p, bufs, start, cnt, pi):
+    """Inverse real pass over row pairs fused into pass A of the dim-0 column
+    four-step (afft_realpair.hip real_icolpair_kernel): S rows 0..h -> Z rows
+    0..h-1 as in _real_rows (inverse); then, for each block n2, the L-point
+    DFT over j of swap(Z[j*h2 + n2]) times W_h^(n2*k1), stored at entry
+    (k1, n2, c) = k1*out.s_j + n2*m + c."""
+    h, m, h2, L = p.rr_h, p.rr_m, p.n_blo, p.L
+    assert h == L * h2
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    rows = np.arange(h + 1).reshape(1, h + 1, 1)
+    c = np.arange(m).reshape(1, 1, m)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    e_in = uin + b * p.in_.s_b + rows * p.in_.s_j + c * (p.in_.s_c or 1)
+    S = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    k = np.arange(h)
+    s0 = S[:, :h, :]
+    s1 = S[:, h - k, :]
+    w = np.exp(-2j * np.pi * k / (2 * h)).reshape(1, h, 1)
+    Z = (s0 + np.conj(s1)) - 1j * (np.conj(s1) - s0) * np.conj(w)
+    Z[:, 0, :] = (S[:, 0, :].real + S[:, h, :].real) + 1j * (S[:, 0, :].real - S[:, h, :].real)
+    X = np.fft.fft(_swap(Z).reshape(cnt, L, h2, m), axis=1)  # row j*h2 + n2 -> bin k1
+    k1 = np.arange(L).reshape(1, L, 1, 1)
+    n2 = np.arange(h2).reshape(1, 1, h2, 1)
+    X = X * np.exp(-2j * np.pi * ((n2 * k1) % h) / h)
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    bo = np.arange(cnt).reshape(cnt, 1, 1, 1)
+    co = np.arange(m).reshape(1, 1, 1, m)
+    e_out = uout + bo * p.out.s_b + k1 * p.out.s_j + n2 * m + co * (p.out.s_c or 1)
+    _scatter(p.out, bufs, e_out, X, f"pass {pi} store")
+
+
 def _real_tr(p, bufs, start, cnt, pi):
     """Transposing forward real pass (afft_aux.hip real_tr_kernel): input
     Y[b][k1][k2] (n1 x n2) holds Z[k1 + n1*k2]; output bins 0..h natural."""
@@ -453,6 +484,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant
                 _real_irowfft(p, bufs, start, cnt, pi)
             elif p.type == 7:
                 _real_colpair(p, bufs, start, cnt, pi)
+            elif p.type == 8:
+                _real_icolpair(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

@@ -139,18 +139,31 @@ def test_plan_model_matches_oracle(kind, dims):
 
 @pytest.mark.parametrize("kind", ["r", "ri"])
 def test_plan_model_real_pass_in_rows(kind):
-    # long real dim 0 with a 1024-point last axis: the real pass over row
-    # pairs runs inside the row FFTs (pass type 6 for c2r; for r2c type 5 on
-    # request, by default type 7, inside the column pass B)
+    # long real dim 0 with a 1024-point last axis: by default the real pass
+    # over row pairs runs inside the column pass B (r2c, type 7) or A (c2r,
+    # type 8); on request inside the row FFTs (type 5, or 6 for c2r)
     for fmt in (IL, SP):
         for batch in (1, 3):
             err, info, passes = model_check(kind, (8192, 1024), fmt, batch)
-            assert [p.type for p in passes] == ([0, 7, 0] if kind == "r" else [6, 0, 0])
+            assert [p.type for p in passes] == ([0, 7, 0] if kind == "r" else [0, 8, 0])
             assert err < 1e-9, (kind, fmt, batch, err)
-            if kind == "r":
-                err, info, passes = model_check(kind, (8192, 1024), fmt, batch, variant={"real_pair": 1})
-                assert [p.type for p in passes] == [0, 0, 5]
-                assert err < 1e-9, (kind, fmt, batch, err)
+            err, info, passes = model_check(kind, (8192, 1024), fmt, batch, variant={"real_pair": 1})
+            assert [p.type for p in passes] == ([0, 0, 5] if kind == "r" else [6, 0, 0])
+            assert err < 1e-9, (kind, fmt, batch, err)
+
+
+@pytest.mark.parametrize("dims", [(8192, 64), (16384, 64), (8192, 2, 64)], ids=lambda d: "x".join(map(str, d)))
+def test_plan_model_inverse_real_pass_in_column_pass_a(dims):
+    # c2r mirror: the inverse real pass over row pairs runs in pass A of the
+    # dim-0 column four-step (blocks n2 and h2 - n2 per work-group), and with
+    # real_pair = -1 as a pass of its own
+    for fmt in (IL, SP):
+        for v in (None, {"real_pair": -1}):
+            err, info, passes = model_check("ri", dims, fmt, 2, variant=v)
+            assert (8 in [p.type for p in passes]) == (v is None), [(p.type, p.L) for p in passes]
+            assert err < 1e-9, (dims, fmt, v, err)
+    err, info, passes = model_check("ri", dims, IL, 1, prec=64)
+    assert 8 in [p.type for p in passes] and err < 1e-9
 
 
 @pytest.mark.parametrize("dims", [(8192, 64), (16384, 64), (65536, 16), (8192, 2, 64)],

@@ -155,6 +155,45 @@ def test_nd_r2c_real_pass_forms(G, shape):
     assert any("real_rows" in d for s_ in seen for d in s_), seen
 
 
+@pytest.mark.parametrize("shape", [(8192, 64), (8192, 1024), (16384, 2048), (8192, 4, 64)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_nd_c2r_real_pass_forms(G, shape):
+    """N-D c2r with a long dim 0: the inverse real pass over row pairs inside
+    the column pass A (default), inside the inverse row FFTs (real_pair = 1,
+    2-D with 1024..8192-point rows) and on its own (-1), each against the
+    oracle, in both formats and a batch of 2."""
+    import torch
+    from fft_amd.afft import Plan
+    n = int(np.prod(shape))
+    hs = G.half(shape)
+    m = int(np.prod(hs))
+    z = np.stack([O.hermitian_input(11 + b, shape) for b in range(2)])
+    ref = [O.irfft(z[b].astype(np.complex128), shape).ravel() for b in range(2)]
+    zf = z.reshape(2, m)
+    seen = set()
+    for fmt in (G.IL, G.SP):
+        p = Plan("ri", shape, 2, format=fmt)
+        for rp in (0, 1, -1):
+            p.set_variant(real_pair=rp)
+            desc = tuple(p.describe())
+            seen.add(desc)
+            say("nd c2r real pass", shape, fmt, rp, desc)
+            dst = G.empty_dev(2 * n)
+            if fmt == G.IL:
+                src = G.to_dev(np.stack([zf.real, zf.imag], -1).astype(np.float32).ravel())
+                p(src, None, dst, None)
+            else:
+                p(G.to_dev(zf.real.astype(np.float32).ravel()), G.to_dev(zf.imag.astype(np.float32).ravel()), dst,
+                  None)
+            torch.cuda.synchronize()
+            y = G.to_host(dst).reshape(2, n)
+            for b in range(2):
+                assert O.rel_l2(y[b].astype(np.float32), ref[b]) <= O.bound(n), (shape, fmt, rp, b)
+        p.close()
+    assert any("real_icolpair" in d for s_ in seen for d in s_), seen
+    assert any("real_rows" in d for s_ in seen for d in s_), seen
+
+
 @pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fp64_path(G, shape):
