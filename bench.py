@@ -306,7 +306,12 @@ def main():
     extra["step_frac_of_copy_ceiling"] = round(hbm_frac_step * HBM_PEAK_GBPS / extra["copy_ceiling_GBps"], 4)
 
     if not args.no_extra:
-        del z
+        # the side workloads run on their own: the headline's plans and
+        # buffers are released first (with them alive, 16 x r2c 2^22 measured
+        # 0.42 instead of 0.40 ms)
+        fwd.close()
+        inv.close()
+        del z, x, y
         torch.cuda.empty_cache()
         # r2c 16 x 2^22 (standard normal)
         nr, br = 1 << 22, 16

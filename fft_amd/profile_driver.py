@@ -67,6 +67,7 @@ def main():
             "r2c_2p20_tr": ("r", (1 << 20,), 64),
             "c2c_256c": ("c", (256, 256, 256), 1),
             "r2c_8192s": ("r", (8192, 8192), 1),
+            "c2r_8192s": ("ri", (8192, 8192), 1),
             "c2c_2p16_4096": ("c", (1 << 16,), 4096),
             "c2c_2p16_4096_mfma": ("c", (1 << 16,), 4096),
         }
@@ -75,6 +76,11 @@ def main():
         if kind == "c":
             x = torch.rand(2 * n * batch, device=dev, generator=g)
             y = torch.empty_like(x)
+        elif kind == "ri":
+            # timing input: random half spectrum (not Hermitian-consistent)
+            hs = (dims[0] // 2 + 1) * int(np.prod(dims[1:]))
+            x = torch.randn(2 * hs * batch, device=dev, generator=g)
+            y = torch.empty(n * batch, device=dev)
         else:
             x = torch.randn(n * batch, device=dev, generator=g)
             hs = (dims[0] // 2 + 1) * int(np.prod(dims[1:]))
