@@ -91,7 +91,7 @@ ND_SHAPES = [(2, 4), (4, 1, 8), (16, 32), (32, 16), (8, 8, 8), (2, 2, 2, 2), (81
              (16384, 8), (4096, 4, 4), (8192, 64),
              # pass B of the dim-0 column four-step with the real pass (64-, 128-
              # and 256-point columns)
-             (16384, 64), (65536, 16),
+             (16384, 64), (65536, 16), (131072, 16),
              # long real dim 0 with 1024..8192-point rows: the real pass inside the
              # column pass B (r2c) or the inverse row FFTs (c2r)
              (8192, 1024), (8192, 2048)]
@@ -194,7 +194,8 @@ def test_nd_c2r_real_pass_forms(G, shape):
     assert any("real_rows" in d for s_ in seen for d in s_), seen
 
 
-@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64)],
+@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64),
+                                   (16384, 64), (65536, 16)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fp64_path(G, shape):
     n = int(np.prod(shape))
