@@ -1805,6 +1805,12 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
       Variant v;
       v.real_pair = P.pass[i].type == PT_REALTR ? 1 : -1;
       cands.push_back(v);
+      if (P.pass[i].type == PT_REALCOLPAIR || P.pass[i].type == PT_REALICOLPAIR) {
+        // and the row-FFT form (2-D with 1024..8192-point rows)
+        Variant w;
+        w.real_pair = 1;
+        cands.push_back(w);
+      }
       break;
     }
   hipEvent_t e0, e1;

@@ -495,3 +495,45 @@ def test_pipelined_plan_graph_replay(G):
         torch.cuda.synchronize()
         assert torch.equal(out, ref), rep
     p.close()
+
+
+def test_autotune_keeps_a_correct_variant_and_caches_it(G, tmp_path, monkeypatch):
+    """afft_gpu_plan_tune times the candidate variants on the caller's
+    buffers, keeps the fastest (which still computes the transform to the
+    bar) and records it in AFFT_GPU_TUNE_CACHE; a new plan of the same shape
+    starts from the recorded variant."""
+    import torch
+    from fft_amd.afft import Plan
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("AFFT_GPU_TUNE_CACHE", str(cache))
+    for kind, shape, batch in (("c", (1 << 20,), 4), ("r", (8192, 1024), 1)):
+        say("autotune", kind, shape, batch)
+        n = int(np.prod(shape))
+        if kind == "c":
+            x = O.complex_input(71, (batch, n))
+            src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
+            ref = [O.fft(x[b].astype(np.complex128)).ravel() for b in range(batch)]
+            m = n
+        else:
+            xr = O.normal(72, batch * n).reshape((batch,) + shape)
+            src = G.to_dev(xr.ravel())
+            ref = [O.rfft(xr[b].astype(np.float64)).ravel() for b in range(batch)]
+            m = int(np.prod(G.half(shape)))
+        p = Plan(kind, shape, batch)
+        dst = G.empty_dev(2 * m * batch)
+        best = p.tune(src, None, dst, None, iters=2)
+        assert best > 0
+        chosen = p.get_variant().as_dict()
+        say("  best", best, chosen, p.describe())
+        p(src, None, dst, None)
+        torch.cuda.synchronize()
+        y = G.to_host(dst)
+        y = (y[0::2] + 1j * y[1::2]).reshape(batch, m)
+        for b in range(batch):
+            assert O.rel_l2(y[b].astype(np.complex64), ref[b]) <= O.bound(n), (kind, shape, chosen, b)
+        q = Plan(kind, shape, batch)
+        assert q.get_variant().as_dict() == chosen
+        q.close()
+        p.close()
+    lines = [ln for ln in cache.read_text().splitlines() if ln.strip()]
+    assert len(lines) >= 2, lines
