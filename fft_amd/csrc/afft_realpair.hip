@@ -1,6 +1,7 @@
 // afft_realpair.hip -- real passes fused into FFT passes: pass B of the
-// four-step 1-D r2c with the real post-pass and the output transposition, and
-// the N-D r2c real pass over row pairs with the first inner-axis row pass.
+// four-step 1-D r2c with the real post-pass and the output transposition; the
+// N-D r2c / c2r real pass over row pairs with the inner-axis row FFTs; and the
+// same pass inside the dim-0 column four-step (r2c: pass B, c2r: pass A).
 //
 // A long r2c (N reals, h = N/2 = n1 * n2 complex points) runs its h-point c2c
 // as a four-step: pass A (n1-point columns with the W_h twiddle) leaves
