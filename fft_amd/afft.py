@@ -18,6 +18,58 @@ from . import _lib
 from ._lib import afft_auto, afft_interleaved, afft_split  # noqa: F401
 
 
+def _half_count(dims) -> int:
+    """Complex elements of a real transform's complex side: dim 0 (after unit
+    dims are dropped) becomes N0/2+1 (fft_internal.hpp:215-219)."""
+    s = [int(d) for d in dims if int(d) != 1] or [1]
+    return (s[0] // 2 + 1) * int(np.prod(s[1:], dtype=np.int64))
+
+
+def required_counts(kind: str, dims, fmt: int, batch: int = 1):
+    """Scalars each of (src_re, src_im, dst_re, dst_im) must hold for `batch`
+    transforms back to back (0: the argument is not read or written).  Real
+    kinds take (src, dst_re, dst_im) and (src_re, src_im, dst); the tuple is
+    in that argument order."""
+    n = int(np.prod([int(d) for d in dims], dtype=np.int64))
+    m = _half_count(dims)
+    il = int(fmt) == afft_interleaved
+    b = int(batch)
+    if kind in ("c", "ci"):
+        return (2 * n * b, 0, 2 * n * b, 0) if il else (n * b, n * b, n * b, n * b)
+    if kind == "r":
+        return (n * b, 2 * m * b, 0) if il else (n * b, m * b, m * b)
+    return (2 * m * b, 0, n * b) if il else (m * b, m * b, n * b)
+
+
+def check_buffers(bufs, counts, dtype):
+    """Rejects, before any launch, torch tensors that are on the host, of the
+    wrong element type, not contiguous or too small for the plan, and a missing
+    buffer the plan needs.  Raw integer addresses are the caller's to get
+    right."""
+    dt = np.dtype(dtype)
+    names = ("first", "second", "third", "fourth")
+    for i, (x, need) in enumerate(zip(bufs, counts)):
+        if need == 0:
+            continue
+        if x is None:
+            raise ValueError(f"the {names[i]} buffer is required ({need} elements) but is None")
+        if isinstance(x, int):
+            if x == 0:
+                raise ValueError(f"the {names[i]} buffer is a null address")
+            continue
+        if not hasattr(x, "data_ptr"):
+            raise TypeError(f"the {names[i]} buffer must be a device tensor or an address, not {type(x).__name__}")
+        if not getattr(x, "is_cuda", False):
+            raise ValueError(f"the {names[i]} buffer is not on the GPU")
+        tdt = str(x.dtype).replace("torch.", "")
+        if tdt != dt.name:
+            raise ValueError(f"the {names[i]} buffer has dtype {tdt}, the plan computes in {dt.name}")
+        if not x.is_contiguous():
+            raise ValueError(f"the {names[i]} buffer is not contiguous")
+        if x.numel() < need:
+            raise ValueError(f"the {names[i]} buffer holds {x.numel()} elements, the plan needs {need}")
+
+
 def _prefix(dtype) -> str:
     dt = np.dtype(dtype)
     if dt == np.float32:
@@ -81,6 +133,7 @@ class complex_transform(_Transform):
 
     def __call__(self, src_re, src_im, dst_re, dst_im):
         self._check()
+        check_buffers((src_re, src_im, dst_re, dst_im), required_counts(self._kind, self.dims, self.format), self.dtype)
         self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re), _lib.ptr(dst_im))
 
 
@@ -90,6 +143,7 @@ class inverse_complex_transform(_Transform):
 
     def __call__(self, src_re, src_im, dst_re, dst_im):
         self._check()
+        check_buffers((src_re, src_im, dst_re, dst_im), required_counts(self._kind, self.dims, self.format), self.dtype)
         self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re), _lib.ptr(dst_im))
 
 
@@ -99,6 +153,7 @@ class real_transform(_Transform):
 
     def __call__(self, src, dst_re, dst_im):
         self._check()
+        check_buffers((src, dst_re, dst_im), required_counts("r", self.dims, self.format), self.dtype)
         self._fn_transform(self.state, _lib.ptr(src), _lib.ptr(dst_re), _lib.ptr(dst_im))
 
 
@@ -108,6 +163,7 @@ class inverse_real_transform(_Transform):
 
     def __call__(self, src_re, src_im, dst):
         self._check()
+        check_buffers((src_re, src_im, dst), required_counts("ri", self.dims, self.format), self.dtype)
         self._fn_transform(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst))
 
 
@@ -146,6 +202,7 @@ class Plan:
         self.batch = int(batch)
         self.format = int(format)
         prec = 32 if np.dtype(dtype) == np.float32 else 64
+        self.dtype = np.dtype(np.float32 if prec == 32 else np.float64)
         nd, arr = _lib.dims_array(self.dims)
         self._mem = ctypes.create_string_buffer(L.afft_gpu_plan_memsize())
         st = L.afft_gpu_plan_create(prec, _lib.KINDS[kind], nd, arr, self.format, self.batch, self._mem)
@@ -153,7 +210,24 @@ class Plan:
             raise RuntimeError(f"plan creation failed: {_lib.last_error()}")
         self.state = st
 
+    def _bufs(self, src_re, src_im, dst_re, dst_im):
+        """Checks the buffers against the plan; the real kinds use the first
+        three of the four arguments as (src, dst_re, dst_im) / (src_re, src_im,
+        dst)."""
+        if not self.state:
+            raise RuntimeError("plan is closed")
+        cnt = required_counts(self.kind, self.dims, self.format, self.batch)
+        if len(cnt) == 3:
+            if self.kind == "r":
+                # execute's (src_re, src_im, dst_re, dst_im) with src_im unused
+                check_buffers((src_re, dst_re, dst_im), cnt, self.dtype)
+            else:
+                check_buffers((src_re, src_im, dst_re), cnt, self.dtype)
+        else:
+            check_buffers((src_re, src_im, dst_re, dst_im), cnt, self.dtype)
+
     def __call__(self, src_re, src_im, dst_re, dst_im, stream=None):
+        self._bufs(src_re, src_im, dst_re, dst_im)
         h = getattr(stream, "cuda_stream", stream)
         r = _lib.lib().afft_gpu_execute(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re),
                                         _lib.ptr(dst_im), h)
@@ -162,6 +236,7 @@ class Plan:
 
     def run_range(self, first, last, src_re, src_im, dst_re, dst_im, stream=None):
         """Enqueues passes [first, last) only (for per-kernel timing)."""
+        self._bufs(src_re, src_im, dst_re, dst_im)
         h = getattr(stream, "cuda_stream", stream)
         r = _lib.lib().afft_gpu_execute_range(self.state, int(first), int(last), _lib.ptr(src_re), _lib.ptr(src_im),
                                               _lib.ptr(dst_re), _lib.ptr(dst_im), h)
@@ -192,6 +267,7 @@ class Plan:
 
     def tune(self, src_re, src_im, dst_re, dst_im, stream=None, iters: int = 3) -> float:
         """Runs the autotuner on these buffers; returns the best ms."""
+        self._bufs(src_re, src_im, dst_re, dst_im)
         h = getattr(stream, "cuda_stream", stream)
         best = ctypes.c_float(0)
         r = _lib.lib().afft_gpu_plan_tune(self.state, _lib.ptr(src_re), _lib.ptr(src_im), _lib.ptr(dst_re),

@@ -1,6 +1,6 @@
 """Builds fft_amd/libafft_mi355x.so from fft_amd/csrc with hipcc for gfx950.
 
-    python -m fft_amd.build        (or fft_amd.build.build())
+    python -m fft_amd.build [--incremental]       (or fft_amd.build.build())
 
 The four translation units compile in parallel; the FP32 and FP64 pass-kernel
 instantiations dominate the build time.
@@ -31,20 +31,37 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _up_to_date(src: str, out: str) -> bool:
+    """True when the object exists and is newer than every file its last
+    compile read (the dependency list hipcc wrote next to it)."""
+    dep = out[:-2] + ".d"
+    if not (os.path.exists(out) and os.path.exists(dep)):
+        return False
+    txt = open(dep).read().replace("\\\n", " ")
+    files = txt.split(":", 1)[1].split() if ":" in txt else []
+    t = os.path.getmtime(out)
+    return all(os.path.exists(f) and os.path.getmtime(f) < t for f in files + [os.path.join(CSRC, src)])
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *FLAGS, "-I", os.path.join(PKG, "..", "include"), "-c", os.path.join(CSRC, src), "-o", out]
+    if incremental and _up_to_date(src, out):
+        return out
+    cmd = [HIPCC, *FLAGS, "-I", os.path.join(PKG, "..", "include"), "-MD", "-MF", out[:-2] + ".d", "-c",
+           os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
     return out
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every translation unit (incremental=True: only those older than
+    their source or a header) and links the library and the harness."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -66,5 +83,5 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

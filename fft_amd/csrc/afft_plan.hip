@@ -134,7 +134,13 @@ struct Variant {
   int row_log2 = 0;   // 1-D four-step: row length n2 = 2^row_log2 (0 = default split)
 };
 
+// A plan lives in the caller's `mem`, as afft's does (fft_internal.hpp:24-106):
+// create constructs it there, nothing else is allocated per plan, and the
+// caller frees `mem` when done.  Its device resources are shared by all plans
+// of the process (twiddle tables in TwCache, scratch in ScratchPool), so a plan
+// whose memory is dropped without a destroy call leaks nothing.
 struct Plan {
+  uint32_t magic = 0;
   Variant var;
   int mode = 0;  // afft_gpu_mode_*
   int prec = 32;
@@ -145,8 +151,7 @@ struct Plan {
   idx_t batch = 1;
   idx_t chunk = 1;
   idx_t scr_per_user[2] = {0, 0};  // complex elements of scratch per user transform
-  void* scr[2] = {nullptr, nullptr};
-  size_t scr_bytes[2] = {0, 0};
+  size_t scr_bytes[2] = {0, 0};    // bytes of scratch 0 / 1 one execute uses (from the pool)
   int npass = 0;
   PassRec pass[kMaxPass];
   hipStream_t stream = nullptr;
@@ -158,23 +163,26 @@ struct Plan {
   int pipe = 0;
 };
 
-struct Header {
-  uint32_t magic;
-  uint32_t prec;
-  Plan* plan;
-};
+// Plans sit at the first kPlanAlign boundary of `mem` (any alignment accepted,
+// like afft's aligned_increment, common.hpp:173-183).
+constexpr size_t kPlanAlign = 64;
 
-bool read_header(const void* state, Header& h) {
+Plan* plan_at(void* mem) {
+  uintptr_t a = ((uintptr_t)mem + kPlanAlign - 1) & ~(uintptr_t)(kPlanAlign - 1);
+  return reinterpret_cast<Plan*>(a);
+}
+
+Plan* get_plan(const void* state) {
   if (!state) {
     set_err("null plan");
-    return false;
+    return nullptr;
   }
-  std::memcpy(&h, state, sizeof(h));
-  if (h.magic != kMagic || !h.plan) {
+  Plan* P = reinterpret_cast<Plan*>(const_cast<void*>(state));
+  if (P->magic != kMagic) {
     set_err("not a plan created by this library");
-    return false;
+    return nullptr;
   }
-  return true;
+  return P;
 }
 
 // ---- device twiddle cache ---------------------------------------------------
@@ -186,6 +194,89 @@ TwCache& twcache() {
   static TwCache* c = new TwCache;  // never destroyed: device memory lives for the process
   return *c;
 }
+
+// ---- device scratch pool ------------------------------------------------------
+// Four-step and real plans need device scratch while they execute.  It comes
+// from one pool per device instead of per-plan allocations, so plans hold no
+// device memory of their own.  Slabs only grow: a request larger than the
+// largest slab adds a slab of at least twice that size (all slabs together stay
+// below twice the largest), and older slabs are kept because a HIP graph
+// captured from an execute may still address them.  A slab is used by one
+// execute at a time in stream order: an execute on another stream than the
+// slab's previous user first waits for that user's last pass (an event), and an
+// execute under graph capture skips that wait and the event (a captured plan's
+// replays must not overlap executes of other plans on other streams).
+struct Slab {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipEvent_t ev = nullptr;
+  hipStream_t last = nullptr;
+  bool used = false;
+};
+struct ScratchPool {
+  std::mutex mu;
+  std::map<int, std::vector<Slab>> slabs;  // by device; sizes increasing
+};
+ScratchPool& scratch_pool() {
+  static ScratchPool* p = new ScratchPool;  // never destroyed: device memory lives for the process
+  return *p;
+}
+
+// Exclusive use of `bytes` of the device's pool on stream `st` for the
+// lifetime of the lease (the pool lock is held; launches go in between).
+class ScratchLease {
+ public:
+  ScratchLease(int dev, size_t bytes, hipStream_t st) : lk_(scratch_pool().mu), st_(st) {
+    if (bytes == 0) {
+      ok_ = true;
+      return;
+    }
+    std::vector<Slab>& v = scratch_pool().slabs[dev];
+    if (v.empty() || v.back().bytes < bytes) {
+      const size_t prev = v.empty() ? 0 : v.back().bytes;
+      size_t want = std::max(bytes, 2 * prev);
+      want = (want + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+      Slab s;
+      hipError_t e = hipMalloc(&s.ptr, want);
+      if (e != hipSuccess && want > bytes) {
+        want = bytes;
+        e = hipMalloc(&s.ptr, want);
+      }
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
+      if (e != hipSuccess) {
+        set_err("scratch allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+        if (s.ptr) (void)hipFree(s.ptr);
+        return;
+      }
+      s.bytes = want;
+      v.push_back(s);
+    }
+    slab_ = &v.back();
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    capturing_ = hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+    if (!capturing_ && slab_->used && slab_->last != st && hipStreamWaitEvent(st, slab_->ev, 0) != hipSuccess) {
+      set_err("scratch: stream wait failed");
+      return;
+    }
+    ok_ = true;
+  }
+  ~ScratchLease() {
+    if (!ok_ || !slab_ || capturing_) return;
+    if (hipEventRecord(slab_->ev, st_) == hipSuccess) {
+      slab_->last = st_;
+      slab_->used = true;
+    }
+  }
+  bool ok() const { return ok_; }
+  void* ptr() const { return slab_ ? slab_->ptr : nullptr; }
+
+ private:
+  std::unique_lock<std::mutex> lk_;
+  hipStream_t st_;
+  Slab* slab_ = nullptr;
+  bool ok_ = false;
+  bool capturing_ = false;
+};
 
 thread_local bool g_dry = false;  // host-only plan inspection: no device calls
 
@@ -1470,17 +1561,22 @@ int a_split(const Plan& P) {
   return (int)(P.chunk / g);
 }
 
+// Scratch 1 starts at this offset into the plan's scratch (scratch 0 first).
+size_t scratch1_offset(const Plan& P) { return (P.scr_bytes[0] + 255) & ~(size_t)255; }
+size_t scratch_total(const Plan& P) { return P.scr_bytes[1] ? scratch1_offset(P) + P.scr_bytes[1] : P.scr_bytes[0]; }
+
+// Launches passes [first, last) of the plan on `st` with its scratch at `scr`
+// (scratch_total(P) bytes).
 template <typename T>
-bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
-             int last = -1) {
+bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first, int last,
+                void* scr) {
   if (last < 0) last = P.npass;
-  Bufs<T> B{(const T*)sre, (const T*)sim, (T*)dre, (T*)dim, {(T*)P.scr[0], (T*)P.scr[1]}};
+  Bufs<T> B{(const T*)sre, (const T*)sim, (T*)dre, (T*)dim,
+            {(T*)scr, P.scr_bytes[1] ? (T*)((char*)scr + scratch1_offset(P)) : nullptr}};
   if (P.format == afft_interleaved) {
     B.src_im = nullptr;
     B.dst_im = nullptr;
   }
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
   if (P.pipe && first == 0 && last == P.npass) {
     // launch k: pass A of group k into scratch half k%2 beside pass B of
     // group k-1 from half (k-1)%2; the launches are ordered on `st`
@@ -1496,7 +1592,7 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
         const idx_t start = k * P.chunk;
         const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
         Bufs<T> Bk = B;
-        Bk.scr[0] = (T*)((char*)P.scr[0] + (k & 1) * half);
+        Bk.scr[0] = (T*)((char*)B.scr[0] + (k & 1) * half);
         if (!fft_params<T>(P.pass[0], Bk, start, cnt, pa, fast_a)) return false;
       }
       if (!fast_a || !fast_b) {
@@ -1510,7 +1606,7 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
         const idx_t start = k * P.chunk;
         const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
         Bufs<T> Bk = B;
-        Bk.scr[0] = (T*)((char*)P.scr[0] + (k & 1) * half);
+        Bk.scr[0] = (T*)((char*)B.scr[0] + (k & 1) * half);
         if (!fft_params<T>(P.pass[1], Bk, start, cnt, pb, fast_b)) return false;
       }
     }
@@ -1538,25 +1634,24 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
   return true;
 }
 
-bool alloc_scratch(Plan& P, size_t elem_bytes) {
+// Launches passes [first, last) of the plan on `st`, its scratch leased from
+// the device's pool.
+template <typename T>
+bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
+             int last = -1) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+  ScratchLease lease(P.device, scratch_total(P), st);
+  if (!lease.ok()) return false;
+  return execute_on<T>(P, sre, sim, dre, dim, st, first, last, lease.ptr());
+}
+
+// Scratch bytes one execute needs (taken from the pool when it runs).
+void size_scratch(Plan& P, size_t elem_bytes) {
   const int pipe = (P.dual && P.batch > P.chunk) ? 1 : 0;
   P.pipe = pipe;
-  for (int i = 0; i < 2; i++) {
-    if (P.scr_per_user[i] == 0) continue;
-    size_t bytes = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes * (i == 0 && pipe ? 2 : 1);
-    if (bytes == P.scr_bytes[i] && P.scr[i]) continue;
-    if (P.scr[i]) (void)hipFree(P.scr[i]);
-    P.scr[i] = nullptr;
-    P.scr_bytes[i] = 0;
-    hipError_t e = hipMalloc(&P.scr[i], bytes);
-    if (e != hipSuccess) {
-      set_err("scratch allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
-      P.scr[i] = nullptr;
-      return false;
-    }
-    P.scr_bytes[i] = bytes;
-  }
-  return true;
+  for (int i = 0; i < 2; i++)
+    P.scr_bytes[i] = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes * (i == 0 && pipe ? 2 : 1);
 }
 
 idx_t default_chunk(const Plan& P, size_t elem_bytes) {
@@ -1607,6 +1702,9 @@ int validate(size_t ndim, const size_t* dim, int format, int impl, idx_t* d, int
 
 bool tcache_get(const Plan& P, Variant& v);  // defined with the autotuner below
 
+// Bytes of `mem` a plan needs: the Plan itself at the first 64-byte boundary.
+constexpr size_t kPlanMem = sizeof(Plan) + kPlanAlign - 1;
+
 template <typename T>
 void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t batch, void* mem, int impl) {
   g_err.clear();
@@ -1626,11 +1724,7 @@ void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t b
     set_err("no GPU visible");
     return nullptr;
   }
-  Plan* P = new (std::nothrow) Plan();
-  if (!P) {
-    set_err("out of host memory");
-    return nullptr;
-  }
+  Plan* P = new (plan_at(mem)) Plan();
   P->prec = sizeof(T) == 4 ? 32 : 64;
   P->kind = kind;
   P->format = format;
@@ -1642,52 +1736,57 @@ void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t b
   if (tcache_get(*P, cached)) P->var = cached;
   Builder<T> b(*P);
   if (!b.build(kind, format, d, nd)) {
-    delete P;
+    P->magic = 0;
     return nullptr;
   }
   P->chunk = default_chunk(*P, sizeof(cx<T>));
-  if (!alloc_scratch(*P, sizeof(cx<T>))) {
-    delete P;
-    return nullptr;
-  }
-  Header h{kMagic, (uint32_t)P->prec, P};
-  std::memcpy(mem, &h, sizeof(h));
-  return mem;
+  size_scratch(*P, sizeof(cx<T>));
+  P->magic = kMagic;
+  return P;
 }
 
+// The plan's device resources are shared (twiddle cache, scratch pool), so
+// destroying a plan only invalidates it; dropping its memory is as good.
 void destroy_impl(void* state) {
   if (!state) return;
-  Header h;
-  std::memcpy(&h, state, sizeof(h));
-  if (h.magic != kMagic || !h.plan) return;
-  Plan* P = h.plan;
-  for (int i = 0; i < 2; i++)
-    if (P->scr[i]) (void)hipFree(P->scr[i]);
-  delete P;
-  std::memset(state, 0, sizeof(h));
+  Plan* P = reinterpret_cast<Plan*>(state);
+  if (P->magic == kMagic) P->magic = 0;
 }
 
-size_t memsize_impl(size_t ndim, const size_t* dim, int format, int impl) {
+// The plan size, or 0 when no plan of this shape can be built (a length
+// beyond the compiled passes); checked by building the pass list on the host.
+template <typename T>
+size_t memsize_impl(int kind, size_t ndim, const size_t* dim, int format, int impl) {
   idx_t d[kMaxDim];
   int nd = 0;
   int st = validate(ndim, dim, format, impl, d, nd);
   if (st == 1) return (size_t)-1;
   if (st != 0) return 0;
-  return sizeof(Header);
+  Plan* P = new (std::nothrow) Plan();
+  if (!P) return 0;
+  P->prec = sizeof(T) == 4 ? 32 : 64;
+  P->kind = kind;
+  P->format = format;
+  P->ndim = nd;
+  for (int i = 0; i < nd; i++) P->dims[i] = d[i];
+  g_dry = true;
+  const bool ok = Builder<T>(*P).build(kind, format, d, nd);
+  g_dry = false;
+  delete P;
+  return ok ? kPlanMem : 0;
 }
 
 template <typename T>
 void transform_impl(void* state, const void* sre, const void* sim, void* dre, void* dim, int kind) {
   g_err.clear();
-  Header h;
-  if (!read_header(state, h)) return;
-  Plan& P = *h.plan;
-  if (P.prec != (int)(sizeof(T) * 8) || P.kind != kind) {
+  Plan* P = get_plan(state);
+  if (!P) return;
+  if (P->prec != (int)(sizeof(T) * 8) || P->kind != kind) {
     set_err("plan kind / precision mismatch");
     return;
   }
-  if (!execute<T>(P, sre, sim, dre, dim, P.stream)) return;
-  hipError_t e = hipStreamSynchronize(P.stream);
+  if (!execute<T>(*P, sre, sim, dre, dim, P->stream)) return;
+  hipError_t e = hipStreamSynchronize(P->stream);
   if (e != hipSuccess) set_err("transform: %s", hipGetErrorString(e));
 }
 
@@ -1763,7 +1862,8 @@ bool rebuild(Plan& P) {
   Builder<T> b(P);
   if (!b.build(P.kind, P.format, P.dims, P.ndim)) return false;
   P.chunk = default_chunk(P, sizeof(cx<T>));
-  return alloc_scratch(P, sizeof(cx<T>));
+  size_scratch(P, sizeof(cx<T>));
+  return true;
 }
 
 bool rebuild_any(Plan& P) { return P.prec == 32 ? rebuild<float>(P) : rebuild<double>(P); }
@@ -1906,7 +2006,7 @@ int afft_supported(int impl) {
 
 #define AFFT_KIND_API(P, T, K, KID)                                                                    \
   size_t afft_##P##_##K##_memsize(size_t ndim, const size_t* dim, int format, int impl) {              \
-    return memsize_impl(ndim, dim, format, impl);                                                     \
+    return memsize_impl<T>(KID, ndim, dim, format, impl);                                             \
   }                                                                                                   \
   afft_##P##_##K##_type* afft_##P##_##K##_create(size_t ndim, const size_t* dim, int format, void* mem, \
                                                  int impl) {                                          \
@@ -1948,7 +2048,7 @@ void afft_64_ri_transform(afft_64_ri_type* s, const double* sr, const double* si
   transform_impl<double>(s, sr, si, dst, nullptr, 3);
 }
 
-size_t afft_gpu_plan_memsize(void) { return sizeof(Header); }
+size_t afft_gpu_plan_memsize(void) { return kPlanMem; }
 
 afft_gpu_plan* afft_gpu_plan_create(int precision, int kind, size_t ndim, const size_t* dim, int format,
                                     size_t batch, void* mem) {
@@ -1969,9 +2069,9 @@ int afft_gpu_execute(afft_gpu_plan* plan, const void* sre, const void* sim, void
 int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void* sre, const void* sim, void* dre,
                            void* dim, void* stream) {
   g_err.clear();
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  Plan& P = *Pp;
   if (last < 0 || last > P.npass) last = P.npass;
   if (first < 0) first = 0;
   bool ok = P.prec == 32 ? execute<float>(P, sre, sim, dre, dim, (hipStream_t)stream, first, last)
@@ -1980,9 +2080,9 @@ int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void*
 }
 
 int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass) {
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  const Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  const Plan& P = *Pp;
   if (pass < 0 || pass >= P.npass) return -1;
   const int split = pass == 0 ? a_split(P) : 1;
   int n = 0;
@@ -1996,35 +2096,30 @@ int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass) {
 void afft_gpu_plan_destroy(afft_gpu_plan* plan) { destroy_impl(plan); }
 
 int afft_gpu_set_stream(void* state, void* stream) {
-  Header h;
-  if (!read_header(state, h)) return -1;
-  h.plan->stream = (hipStream_t)stream;
+  Plan* P = get_plan(state);
+  if (!P) return -1;
+  P->stream = (hipStream_t)stream;
   return 0;
 }
 
 int afft_gpu_plan_set_chunk(afft_gpu_plan* plan, size_t chunk) {
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  Plan& P = *Pp;
   size_t eb = P.prec == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
   idx_t c = chunk == 0 ? default_chunk(P, eb) : (idx_t)chunk;
   if (c > P.batch) c = P.batch;
   if (c < 1) c = 1;
-  idx_t old = P.chunk;
   P.chunk = c;
-  if (!alloc_scratch(P, eb)) {
-    P.chunk = old;
-    alloc_scratch(P, eb);
-    return -1;
-  }
+  size_scratch(P, eb);
   return 0;
 }
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   g_err.clear();
-  Header h;
-  if (!read_header(plan, h) || !v) return -1;
-  Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp || !v) return -1;
+  Plan& P = *Pp;
   Variant old = P.var;
   P.var.col_tile = v->col_tile;
   P.var.rowt_tile = v->rowt_tile;
@@ -2047,9 +2142,9 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
 
 int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode) {
   g_err.clear();
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  Plan& P = *Pp;
   if (mode != afft_gpu_mode_fma && mode != afft_gpu_mode_mfma_f16 && mode != afft_gpu_mode_mfma_bf16) {
     set_err("unknown mode %d", mode);
     return -1;
@@ -2075,9 +2170,9 @@ int afft_gpu_plan_set_mode(afft_gpu_plan* plan, int mode) {
 }
 
 int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
-  Header h;
-  if (!read_header(plan, h) || !v) return -1;
-  const Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp || !v) return -1;
+  const Plan& P = *Pp;
   v->col_tile = P.var.col_tile;
   v->rowt_tile = P.var.rowt_tile;
   v->stream_io = P.var.stream_io;
@@ -2095,9 +2190,9 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
 int afft_gpu_plan_tune(afft_gpu_plan* plan, const void* sre, const void* sim, void* dre, void* dim, void* stream,
                        int iters, float* best_ms) {
   g_err.clear();
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  Plan& P = *Pp;
   if (iters < 1) iters = 3;
   bool ok = P.prec == 32 ? tune_impl<float>(P, sre, sim, dre, dim, (hipStream_t)stream, iters, best_ms)
                          : tune_impl<double>(P, sre, sim, dre, dim, (hipStream_t)stream, iters, best_ms);
@@ -2105,21 +2200,19 @@ int afft_gpu_plan_tune(afft_gpu_plan* plan, const void* sre, const void* sim, vo
 }
 
 int afft_gpu_plan_num_passes(const afft_gpu_plan* plan) {
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  return h.plan->npass;
+  Plan* P = get_plan(plan);
+  return P ? P->npass : -1;
 }
 
 size_t afft_gpu_plan_scratch_bytes(const afft_gpu_plan* plan) {
-  Header h;
-  if (!read_header(plan, h)) return 0;
-  return h.plan->scr_bytes[0] + h.plan->scr_bytes[1];
+  Plan* P = get_plan(plan);
+  return P ? scratch_total(*P) : 0;
 }
 
 int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t len) {
-  Header h;
-  if (!read_header(plan, h)) return -1;
-  const Plan& P = *h.plan;
+  Plan* Pp = get_plan(plan);
+  if (!Pp) return -1;
+  const Plan& P = *Pp;
   if (i < 0 || i >= P.npass || !buf) return -1;
   const PassRec& r = P.pass[i];
   static const char* ep[] = {"plain", "tw4", "r2c", "c2r", "c2r+tw4"};

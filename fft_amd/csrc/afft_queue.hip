@@ -84,11 +84,18 @@ struct QTables {
   const cx<float>* real[kQMaxLog2 + 1];   // W_{2L}^k, by log2 L
 };
 
+// Written by prepare, read by every launch: a launch (or a replay of a captured
+// one) runs the task list of the latest prepare.
+struct QListHdr {
+  int list_off[kXcd + 2];  // kXcd chain lists, then the group list, then the end
+  int ndeps;               // step counters to clear (4 per chain job)
+};
+
 struct QArgs {
   const QTask* tasks;  // kXcd chain lists, then the shared group list
   const unsigned long long* ptrs;
   int* counter;        // list l pops through counter[l * kCtr]
-  int list_off[kXcd + 2];
+  const QListHdr* hdr;
   int* deps;       // 4 step counters per chain job
   int* status;     // set to 1 when a dependency wait hits the spin limit
   long long spin_limit;
@@ -300,14 +307,15 @@ __global__ void __launch_bounds__(kQThreads, GROUPS) queue_kernel(QArgs a) {
   const int lists[2] = {xcc_id(), kXcd};
   for (int li = 0; li < 2; li++) {
     const int l = lists[li];
-    const int n = a.list_off[l + 1] - a.list_off[l];
+    const int off = a.hdr->list_off[l];
+    const int n = a.hdr->list_off[l + 1] - off;
     for (;;) {
       if (tid == 0) s_task = atomicAdd(a.counter + l * kCtr, 1);
       __syncthreads();
       const int ti = s_task;
       __syncthreads();
       if (ti >= n) break;
-      const QTask tk = a.tasks[a.list_off[l] + ti];
+      const QTask tk = a.tasks[off + ti];
       if (tk.type != QT_GROUP)
         q_chain(a, tk, tid, lds);
       else
@@ -319,10 +327,11 @@ __global__ void __launch_bounds__(kQThreads, GROUPS) queue_kernel(QArgs a) {
 
 // Clears the task counter, the status flag and the chain step counters (a
 // kernel, not a memset, so a captured graph replays it reliably).
-__global__ void queue_reset_kernel(int* counter, int* status, int* deps, int ndeps) {
+__global__ void queue_reset_kernel(int* counter, int* status, int* deps, const QListHdr* hdr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i <= kXcd) counter[i * kCtr] = 0;
   if (i == 0) *status = 0;
+  const int ndeps = hdr->ndeps;
   for (int k = i; k < ndeps; k += gridDim.x * blockDim.x) deps[k] = 0;
 }
 
@@ -379,9 +388,12 @@ struct Queue {
   unsigned long long* d_ptrs = nullptr;
   size_t cap_ptrs = 0;
   int* d_counter = nullptr;  // list counters at [l * kCtr], l = 0..kXcd; status at [(kXcd + 1) * kCtr]
-  int list_off[kXcd + 2] = {};
+  QListHdr hdr{};
+  QListHdr* d_hdr = nullptr;
   int* d_deps = nullptr;
   size_t cap_deps = 0;
+  hipEvent_t done = nullptr;  // recorded after each eager launch; prepare waits for it
+  bool launched = false;
   cx<float>* d_scratch = nullptr;
   size_t cap_scratch = 0;  // complex elements
   QTables tab{};
@@ -417,6 +429,11 @@ Queue* get_queue(const void* mem) {
 
 void free_queue(Queue* q) {
   if (!q) return;
+  if (q->done) {
+    (void)hipEventSynchronize(q->done);
+    (void)hipEventDestroy(q->done);
+  }
+  (void)hipFree(q->d_hdr);
   (void)hipFree(q->d_tasks);
   (void)hipFree(q->d_ptrs);
   (void)hipFree(q->d_counter);
@@ -449,8 +466,21 @@ afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem)
   // persistent grid: groups_per_cu 256-thread groups per CU
   q->ncu = ncu;
   q->grid = ncu * q->groups_per_cu;
-  // task, pointer, counter and scratch arrays grow in prepare as needed
+  // The task, pointer and dependency arrays are sized for max_jobs here, so
+  // their addresses never change (a captured launch keeps reading them); only
+  // the chain jobs' scratch grows in prepare.
+  int max_chain_tasks = 1;
+  for (int lgm = 13; lgm <= max_log2n; lgm++) {
+    const int rt = (int)((((size_t)1 << lgm) / 2 + 1 + kRealTile - 1) / kRealTile);
+    max_chain_tasks = std::max(max_chain_tasks, rt + q_chain_a_tiles(lgm) + (1 << (lgm - 8)) / 16);
+  }
   hipError_t e = hipMalloc(&q->d_counter, sizeof(int) * kCtr * (kXcd + 2));
+  if (e == hipSuccess) e = hipMalloc(&q->d_hdr, sizeof(QListHdr));
+  if (e == hipSuccess) e = hipMemset(q->d_hdr, 0, sizeof(QListHdr));
+  if (e == hipSuccess) e = q_reserve(q->d_tasks, q->cap_tasks, max_jobs * (size_t)max_chain_tasks);
+  if (e == hipSuccess) e = q_reserve(q->d_ptrs, q->cap_ptrs, 4 * max_jobs);
+  if (e == hipSuccess) e = q_reserve(q->d_deps, q->cap_deps, 4 * max_jobs);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&q->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     queue_set_err(hipGetErrorString(e));
     free_queue(q);
@@ -496,6 +526,11 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
     queue_set_err("queue: too many jobs for this queue");
     return -1;
   }
+  // the previous eager launch may still read the arrays rewritten below
+  if (q->launched && hipEventSynchronize(q->done) != hipSuccess) {
+    queue_set_err("queue: waiting for the previous launch failed");
+    return -1;
+  }
   // bucket by (kind, complex length)
   std::vector<std::vector<size_t>> small((size_t)4 * 13);
   std::vector<size_t> big;
@@ -537,7 +572,6 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
     scratch_elems += (size_t)1 << lgm;
   }
   hipError_t e0 = q_reserve(q->d_scratch, q->cap_scratch, std::max<size_t>(scratch_elems, 1));
-  if (e0 == hipSuccess) e0 = q_reserve(q->d_deps, q->cap_deps, std::max<size_t>(4 * big.size(), 1));
   if (e0 != hipSuccess) {
     queue_set_err(hipGetErrorString(e0));
     return -1;
@@ -564,7 +598,7 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
   }
   // per XCD list: step s of its k-th job is listed at round k + s*kLag
   for (int x = 0; x < kXcd; x++) {
-    q->list_off[x] = (int)q->h_tasks.size();
+    q->hdr.list_off[x] = (int)q->h_tasks.size();
     const auto& js = on_xcd[x];
     for (size_t round = 0; !js.empty() && round < js.size() + 3 * (size_t)kLag; round++)
       for (int st = 0; st < 4; st++) {
@@ -588,7 +622,7 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
         }
       }
   }
-  q->list_off[kXcd] = (int)q->h_tasks.size();
+  q->hdr.list_off[kXcd] = (int)q->h_tasks.size();
   q->nchain = (int)nb;
   for (int lgm = 12; lgm >= 0; lgm--)
     for (int kind = 0; kind < 4; kind++) {
@@ -609,9 +643,13 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
       }
     }
   q->ntasks = (int)q->h_tasks.size();
-  q->list_off[kXcd + 1] = q->ntasks;
-  hipError_t e = q_reserve(q->d_tasks, q->cap_tasks, std::max<size_t>(q->h_tasks.size(), 1));
-  if (e == hipSuccess) e = q_reserve(q->d_ptrs, q->cap_ptrs, std::max<size_t>(q->h_ptrs.size(), 1));
+  q->hdr.list_off[kXcd + 1] = q->ntasks;
+  q->hdr.ndeps = 4 * q->nchain;
+  if (q->h_tasks.size() > q->cap_tasks || q->h_ptrs.size() > q->cap_ptrs || (size_t)q->hdr.ndeps > q->cap_deps) {
+    queue_set_err("queue: internal: task list exceeds the arrays sized at create");
+    return -1;
+  }
+  hipError_t e = hipMemcpy(q->d_hdr, &q->hdr, sizeof(QListHdr), hipMemcpyHostToDevice);
   if (e == hipSuccess && q->ntasks) {
     e = hipMemcpy(q->d_tasks, q->h_tasks.data(), sizeof(QTask) * q->h_tasks.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -634,14 +672,14 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
   if (q->ntasks == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   static_assert(kQLds * 4 <= 160 * 1024, "four groups per CU must fit the LDS");
-  const int ndeps = 4 * q->nchain;
-  hipLaunchKernelGGL(queue_reset_kernel, dim3((unsigned)std::max(1, std::min(64, (ndeps + 255) / 256))), dim3(256),
-                     0, st, q->d_counter, q->d_counter + (kXcd + 1) * kCtr, q->d_deps, ndeps);
+  const size_t ndeps_max = q->cap_deps;
+  hipLaunchKernelGGL(queue_reset_kernel, dim3((unsigned)std::max<size_t>(1, std::min<size_t>(64, (ndeps_max + 255) / 256))),
+                     dim3(256), 0, st, q->d_counter, q->d_counter + (kXcd + 1) * kCtr, q->d_deps, q->d_hdr);
   QArgs a{};
   a.tasks = q->d_tasks;
   a.ptrs = q->d_ptrs;
   a.counter = q->d_counter;
-  for (int l = 0; l < kXcd + 2; l++) a.list_off[l] = q->list_off[l];
+  a.hdr = q->d_hdr;
   a.deps = q->d_deps;
   a.status = q->d_counter + (kXcd + 1) * kCtr;
   a.spin_limit = q->spin_limit;
@@ -656,6 +694,16 @@ int afft_gpu_queue_launch(afft_gpu_queue* qh, void* stream) {
   if (e != hipSuccess) {
     queue_set_err(hipGetErrorString(e));
     return -1;
+  }
+  // an eager launch is remembered so that the next prepare waits for it; a
+  // captured one is not (the caller finishes its replays before a prepare)
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+    if (hipEventRecord(q->done, st) != hipSuccess) {
+      queue_set_err("queue: event record failed");
+      return -1;
+    }
+    q->launched = true;
   }
   return 0;
 }

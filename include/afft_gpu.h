@@ -36,7 +36,8 @@ enum
 
 typedef struct afft_gpu_plan_struct afft_gpu_plan;
 
-/* Bytes of host memory a plan header needs (any alignment). */
+/* Bytes of host memory a plan needs (any alignment).  The plan lives in that
+ * memory, as afft's does; the caller frees it when done. */
 size_t afft_gpu_plan_memsize(void);
 
 /* A batch of `batch` transforms of shape dim[0..ndim) laid out back to back:
@@ -64,8 +65,9 @@ int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void*
  * batch), or -1. */
 int afft_gpu_plan_launches(const afft_gpu_plan* plan, int pass);
 
-/* Releases the plan's device resources (the header memory stays the
- * caller's). */
+/* Invalidates the plan.  Plans hold no device memory of their own (twiddle
+ * tables and scratch are shared by the process, see afft_gpu_plan_create), so
+ * freeing the plan memory without this call leaks nothing. */
 void afft_gpu_plan_destroy(afft_gpu_plan* plan);
 
 /* Works for a plan made by any afft_XX_Y_create of fft.h as well: the stream
@@ -143,8 +145,13 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v);
 int afft_gpu_plan_tune(afft_gpu_plan* plan, const void* src_re, const void* src_im, void* dst_re, void* dst_im,
                        void* stream, int iters, float* best_ms);
 
-/* Plan introspection: number of kernel launches per execute, and the scratch
- * bytes held on the device. */
+/* Plan introspection: number of kernel launches per execute, and the bytes
+ * of the process-wide scratch pool one execute uses.  Scratch comes from one
+ * pool per device that only grows (to at most twice the largest request) and
+ * is shared by all plans: executes on one stream run in order; an execute on
+ * another stream waits for the pool's previous user; executes under HIP graph
+ * capture skip that wait, so replays of a captured plan must not overlap
+ * executes of other plans on other streams. */
 int afft_gpu_plan_num_passes(const afft_gpu_plan* plan);
 size_t afft_gpu_plan_scratch_bytes(const afft_gpu_plan* plan);
 /* Writes a one-line description of pass i (kernel shape) into buf. */
@@ -174,13 +181,20 @@ size_t afft_gpu_queue_memsize(void);
 /* A queue for up to max_jobs jobs of sizes up to 2^max_log2n (<= 2^16). */
 afft_gpu_queue* afft_gpu_queue_create(size_t max_jobs, int max_log2n, void* mem);
 /* Groups the jobs into tasks and uploads the task list (synchronous host
- * work; grows the queue's device arrays and the scratch of the long jobs as
- * needed).  Jobs of complex length <= 4096 run in groups of one tile; longer
- * ones as chains of tiles (real pre-pass, four-step pass A, pass B, real
- * post-pass) spread over the whole GPU.  Returns the number of tasks, or -1. */
+ * work).  It first waits for the previous eager launch of this queue; the
+ * task, pointer and counter arrays were sized for max_jobs at create, and only
+ * the scratch of the long jobs grows here.  Jobs of complex length <= 4096 run
+ * in groups of one tile; longer ones as chains of tiles (real pre-pass,
+ * four-step pass A, pass B, real post-pass) spread over the whole GPU.
+ * Returns the number of tasks, or -1. */
 int afft_gpu_queue_prepare(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs);
 /* Runs the prepared task list in one persistent launch on `stream`.  Issues
- * kernels only, so it can be captured in a HIP graph and replayed. */
+ * kernels only, so it can be captured in a HIP graph and replayed.  A launch
+ * and every replay of a captured one run the task list of the LATEST prepare
+ * (the list offsets and counts are read from device memory at run time): a
+ * prepare between replays changes what the graph computes, and the caller
+ * must let replays finish before calling prepare (prepare only waits for
+ * eager launches). */
 int afft_gpu_queue_launch(afft_gpu_queue* q, void* stream);
 /* prepare + launch. */
 int afft_gpu_queue_run(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs, void* stream);

@@ -7,11 +7,13 @@
  *   - every src/dst pointer is a DEVICE pointer (hipMalloc'd or equivalent);
  *     floats must be 4-byte aligned (doubles 8-byte), any other alignment is
  *     accepted;
- *   - `mem` (>= memsize bytes, any alignment, host memory) receives a small
- *     host-side plan header; the twiddle tables and scratch the plan needs live
- *     on the GPU and are released by the afft_XX_Y_destroy calls declared at
- *     the end of this header (additions: afft has nothing to release because
- *     its whole plan lives in `mem`);
+ *   - `mem` (>= memsize bytes, any alignment, host memory) holds the whole
+ *     plan, as in afft (fft_internal.hpp:24-106): the caller frees it when
+ *     done and nothing else needs releasing.  The twiddle tables and the
+ *     scratch a plan uses on the GPU are shared by all plans of the process
+ *     (a per-device cache and a per-device scratch pool that only grows), so
+ *     dropping `mem` leaks nothing.  The afft_XX_Y_destroy calls declared at
+ *     the end of this header only invalidate a plan;
  *   - `impl` accepts afft_auto and afft_gpu (see afft_gpu.h); the CPU ISA ids
  *     afft_scalar..afft_neon are not provided by this library, so memsize
  *     returns 0 and create returns NULL for them (afft's convention for an
@@ -124,7 +126,8 @@ int afft_supported(int impl);
 /* replaces afft_alignment (fft.h:84, fft.cpp:308): 64 */
 extern const size_t afft_alignment;
 
-/* Additions: release the device resources of a plan.  `mem` stays owned by
+/* Additions: invalidate a plan (later transform calls on it fail).  Plans
+ * own no device memory, so calling these is optional; `mem` stays owned by
  * the caller, as in afft.  Passing NULL is a no-op. */
 void afft_32_c_destroy(afft_32_c_type* state);
 void afft_32_ci_destroy(afft_32_ci_type* state);

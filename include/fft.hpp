@@ -5,8 +5,9 @@
 // inverse_real_transform, each over float / double and a format), so code
 // written against afft's C++ API compiles unchanged.  Differences:
 //   - the buffers passed to operator() are device pointers;
-//   - the destructor also releases the plan's device resources (the matching
-//     afft_XX_Y_destroy), which afft has no need for;
+//   - the destructor also calls the matching afft_XX_Y_destroy, which only
+//     invalidates the plan (plans own no device memory; their twiddle tables
+//     and scratch are shared by the process);
 //   - a plan can be moved to another HIP stream with set_stream().
 #ifndef AFFT_MI355X_FFT_HPP
 #define AFFT_MI355X_FFT_HPP

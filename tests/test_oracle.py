@@ -184,3 +184,111 @@ def test_oracle_against_reference_build_ndim_and_split():
             h = O.hermitian_input(3, shape)
             w = R.c2r(h, shape, fmt=fmt, impl=O.AFFT_AVX2)
             assert O.rel_l2(w, O.irfft(h.astype(np.complex128), shape)) <= O.bound(n)
+
+
+# ---- the error metrics --------------------------------------------------------
+# rel-L2 = ||y - ref|| / ||ref|| and the reference harness metric
+# sqrt(sum (d1 - d0)^2 / sum (d0^2 + d1^2)) (testing.cpp:1113-1127) decide
+# every pass / fail below, so they are pinned by cases whose value the
+# mathematics fixes exactly, and each check is shown to reject the plausible
+# slips (a dropped square root, numerator and denominator swapped, a
+# difference that is never accumulated, a denominator over the candidate
+# instead of the reference).
+
+
+def _exact_pair(n=4096):
+    """ref: small integers (exact in float32 and float64); y = ref * (1 + 2^-10),
+    also exact in float32.  rel-L2(y, ref) is then exactly 2^-10."""
+    rng = np.random.default_rng(17)
+    ref = rng.integers(-1000, 1000, size=2 * n).astype(np.float64)
+    ref[ref == 0] = 7.0
+    y = (ref * (1.0 + 2.0 ** -10)).astype(np.float32)
+    assert np.array_equal(y.astype(np.float64), ref * (1.0 + 2.0 ** -10))
+    c = lambda a: a.view(np.complex64 if a.dtype == np.float32 else np.complex128)
+    return c(y), c(ref)
+
+
+def _rel_l2_checks(rel):
+    """Properties of rel-L2 that fix its value; `rel(y, ref)` takes a complex64
+    or complex128 candidate and a complex128 reference."""
+    y, ref = _exact_pair()
+    assert abs(rel(y, ref) - 2.0 ** -10) <= 1e-15
+    assert rel(ref.astype(np.complex64), ref) == 0.0
+    assert abs(rel(np.zeros_like(y), ref) - 1.0) <= 1e-15
+    assert abs(rel((-ref).astype(np.complex64), ref) - 2.0) <= 1e-15
+    assert abs(rel((3 * ref).astype(np.complex64), ref) - 2.0) <= 1e-15
+    # one component off by d: sqrt(d^2 / sum ref^2)
+    z = ref.astype(np.complex64).copy()
+    z[5] += np.float32(0.5)
+    assert abs(rel(z, ref) - 0.5 / np.sqrt(np.sum(np.abs(ref) ** 2))) <= 1e-15
+    # scale invariance in the reference's magnitude
+    assert abs(rel((1024 * y).astype(np.complex64), 1024 * ref) - 2.0 ** -10) <= 1e-15
+    # random data against numpy's norms
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(3000) + 1j * rng.standard_normal(3000)
+    c = (r + 1e-3 * (rng.standard_normal(3000) + 1j * rng.standard_normal(3000))).astype(np.complex64)
+    want = np.linalg.norm(c.astype(np.complex128) - r) / np.linalg.norm(r)
+    assert abs(rel(c, r) - want) <= 1e-12 * want
+
+
+def test_rel_l2_pinned():
+    _rel_l2_checks(O.rel_l2)
+    # the FP64-candidate entry point (orc_rel_l2_d) on the same cases
+    _rel_l2_checks(lambda y, ref: O.rel_l2(np.asarray(y).astype(np.complex128), ref))
+
+
+def test_rel_l2_checks_reject_planted_mistakes():
+    """Each plausible slip, planted on the CPU around the oracle's own metric,
+    fails the checks above."""
+    ident = lambda y, ref: O.rel_l2(y, ref)
+    mutants = {
+        "dropped sqrt": lambda y, ref: ident(y, ref) ** 2,
+        "num/den swapped": lambda y, ref: 1.0 / ident(y, ref) if ident(y, ref) else np.inf,
+        "never accumulated": lambda y, ref: 0.0,
+        "normalised by the candidate": lambda y, ref: (
+            ident(y, ref) * np.linalg.norm(ref) / max(np.linalg.norm(np.asarray(y, np.complex128)), 1e-300)),
+        "abs error (no denominator)": lambda y, ref: ident(y, ref) * np.linalg.norm(ref),
+    }
+    for name, m in mutants.items():
+        with pytest.raises(AssertionError):
+            _rel_l2_checks(m)
+        assert name
+
+
+def _harness_checks(h):
+    """testing.cpp:1113-1127: sqrt(sum (d1-d0)^2 / sum (d0^2 + d1^2)) with d0
+    the reference output and d1 the candidate."""
+    y, ref = _exact_pair()
+    e = 2.0 ** -10
+    # d1 = d0 (1 + e): e / sqrt(1 + (1 + e)^2)
+    assert abs(h(y, ref) - e / np.sqrt(1.0 + (1.0 + e) ** 2)) <= 1e-15
+    assert h(ref.astype(np.complex64), ref) == 0.0
+    assert abs(h(np.zeros_like(y), ref) - 1.0) <= 1e-15
+    # d1 = -d0: sqrt(4 / 2)
+    assert abs(h((-ref).astype(np.complex64), ref) - np.sqrt(2.0)) <= 1e-15
+    # d1 = 3 d0: sqrt(4 / 10)
+    assert abs(h((3 * ref).astype(np.complex64), ref) - np.sqrt(0.4)) <= 1e-15
+    # small errors: the harness metric is rel-L2 / sqrt(2) to first order
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(2000) + 1j * rng.standard_normal(2000)
+    c = (r + 1e-4 * (rng.standard_normal(2000) + 1j * rng.standard_normal(2000))).astype(np.complex64)
+    assert abs(h(c, r) * np.sqrt(2.0) / O.rel_l2(c, r) - 1.0) < 1e-3
+
+
+def test_harness_error_pinned():
+    _harness_checks(O.harness_error)
+
+
+def test_harness_checks_reject_planted_mistakes():
+    ident = lambda y, ref: O.harness_error(y, ref)
+    mutants = {
+        "dropped sqrt": lambda y, ref: ident(y, ref) ** 2,
+        "rel-L2 instead": lambda y, ref: O.rel_l2(y, ref),
+        "never accumulated": lambda y, ref: 0.0,
+        "denominator without d1": lambda y, ref: O.rel_l2(y, ref),
+        "num/den swapped": lambda y, ref: 1.0 / ident(y, ref) if ident(y, ref) else np.inf,
+    }
+    for name, m in mutants.items():
+        with pytest.raises(AssertionError):
+            _harness_checks(m)
+        assert name
