@@ -37,6 +37,7 @@
 #include "../../include/afft_gpu.h"
 #include "../../include/fft.h"
 #include "afft_aux.h"
+#include "afft_fused.h"
 #include "afft_mfma.h"
 #include "afft_registry.h"
 
@@ -132,6 +133,9 @@ struct Variant {
   int real_pair = 0;  // fused real passes: 1 = all (also 1-D r2c pass B), 0 (default) = the N-D row-pair one, -1 = none
   int a_split = 0;    // two-pass plans: launches of pass A per group (0 / 1: one)
   int row_log2 = 0;   // 1-D four-step: row length n2 = 2^row_log2 (0 = default split)
+  int fused = 0;      // 1: both four-step passes in one persistent launch where supported
+  int fused_slots = 0;  // ring slots per XCD of the fused launch (0 = 3)
+  int fused_lag = 0;    // rounds between pass A and pass B of a transform (0 = 1)
 };
 
 // A plan lives in the caller's `mem`, as afft's does (fft_internal.hpp:24-106):
@@ -161,6 +165,9 @@ struct Plan {
   // 0 then holds two group buffers, used alternately
   const void* dual = nullptr;
   int pipe = 0;
+  // both passes of a 1-D FP32 c2c four-step in one persistent launch
+  // (afft_fused.hip): ring slots per XCD and the pass-B lag, 0 = off
+  int fused_slots = 0, fused_lag = 0;
 };
 
 // Plans sit at the first kPlanAlign boundary of `mem` (any alignment accepted,
@@ -924,8 +931,14 @@ struct Builder {
     bool r = build_passes(kind, format, d, nd);
     for (int i = 0; i < P.npass; i++) {
       PassRec& q = P.pass[i];
+      // Streaming (non-temporal) accesses for the caller's input and for the
+      // final write of the output; a pass whose output a later pass reads
+      // again (the outer axes of N-D plans run in place on dst) stores
+      // through the caches, so the array can stay in the MALL in between.
+      bool reread = false;
+      for (int j = i + 1; j < P.npass; j++) reread = reread || P.pass[j].in.role == q.out.role;
       q.in_nt = P.var.stream_io && q.in.role == R_SRC;
-      q.out_nt = P.var.stream_io && q.out.role == R_DST;
+      q.out_nt = P.var.stream_io && q.out.role == R_DST && !reread;
       q.xcd = P.var.xcd_order;
       q.mfma = 0;
       if (P.mode != afft_gpu_mode_fma && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
@@ -936,7 +949,29 @@ struct Builder {
       }
     }
     if (r && ok) pick_dual();
+    if (r && ok) pick_fused();
     return r && ok;
+  }
+
+  // Batched 1-D FP32 c2c four-step plans whose two pass shapes have a fused
+  // kernel run both passes in one persistent launch (afft_fused.hip) when the
+  // variant asks for it.
+  void pick_fused() {
+    P.fused_slots = P.fused_lag = 0;
+    if (P.var.fused <= 0 || sizeof(T) != 4 || P.npass != 2 || P.dual || P.mode != afft_gpu_mode_fma ||
+        P.kind > 1 || P.ndim != 1 || P.format != afft_interleaved)
+      return;
+    const PassRec& a = P.pass[0];
+    const PassRec& b = P.pass[1];
+    if (a.type != PT_FFT || b.type != PT_FFT || !a.colmap || a.epi != EPI_TW4 || a.store_t || b.colmap ||
+        !b.store_t || b.epi != EPI_PLAIN || a.in.role != R_SRC || a.out.role != R_SCR || b.in.role != R_SCR ||
+        b.out.role != R_DST || a.mfma || b.mfma || a.n_blo != 1 || b.n_blo != 1 || a.fold_user || b.fold_user ||
+        (idx_t)a.L * b.L != P.dims[0] || a.n_col != b.L || b.n_col != a.L)
+      return;
+    if (!fused_supported(a.L, a.CT, a.sv, a.hx, b.L, b.CT, b.sv, b.hx)) return;
+    P.fused_slots = P.var.fused_slots > 0 ? P.var.fused_slots : 3;
+    P.fused_lag = P.var.fused_lag > 0 ? P.var.fused_lag : 1;
+    if (P.fused_lag >= P.fused_slots) P.fused_lag = P.fused_slots - 1;
   }
 
   // Two-pass four-step plans whose pass shapes have a fused kernel run as
@@ -1577,6 +1612,32 @@ bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim,
     B.src_im = nullptr;
     B.dst_im = nullptr;
   }
+  if (P.fused_slots && first == 0 && last == P.npass) {
+    // both passes in one persistent launch over the whole batch
+    const PassRec& a = P.pass[0];
+    const PassRec& b = P.pass[1];
+    FusedArgs f{};
+    f.src = (const float*)B.src_re;
+    f.dst = (float*)B.dst_re;
+    f.ring = (float*)B.scr[0];
+    f.src_stride = 2 * a.in.s_u;
+    f.dst_stride = 2 * b.out.s_u;
+    f.stage_a = (const cx<float>*)a.stage_tw;
+    f.stage_b = (const cx<float>*)b.stage_tw;
+    f.tw4_hi = (const cx<float>*)a.tw4_hi;
+    f.tw4_lo = (const cx<float>*)a.tw4_lo;
+    f.tw4_bits = a.tw4_bits;
+    f.swap = a.swap_in;
+    f.in_nt = a.in_nt;
+    f.out_nt = b.out_nt;
+    f.batch = (int)P.batch;
+    f.slots = P.fused_slots;
+    f.lag = P.fused_lag;
+    f.ctr = (int*)((char*)B.scr[0] + fused_ring_bytes(P));
+    f.spin_limit = 1LL << 26;
+    HIP_OK(fused_launch(f, a.L, a.CT, a.sv, a.hx, b.L, b.CT, b.sv, b.hx, st));
+    return true;
+  }
   if (P.pipe && first == 0 && last == P.npass) {
     // launch k: pass A of group k into scratch half k%2 beside pass B of
     // group k-1 from half (k-1)%2; the launches are ordered on `st`
@@ -1634,24 +1695,84 @@ bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim,
   return true;
 }
 
+// Bytes of each input array and each output array one execute touches
+// (split format: per re / im array).
+void io_bytes(const Plan& P, size_t elem, size_t& in_b, size_t& out_b) {
+  idx_t n = 1, inner = 1;
+  for (int i = 0; i < P.ndim; i++) n *= P.dims[i];
+  for (int i = 1; i < P.ndim; i++) inner *= P.dims[i];
+  const idx_t m = (P.dims[0] / 2 + 1) * inner;  // complex side of a real transform
+  const idx_t c = P.format == afft_interleaved ? 2 : 1;
+  idx_t in_s = c * n, out_s = c * n;
+  if (P.kind == 2) {
+    in_s = n;
+    out_s = c * m;
+  } else if (P.kind == 3) {
+    in_s = c * m;
+    out_s = n;
+  }
+  in_b = (size_t)(in_s * P.batch) * elem;
+  out_b = (size_t)(out_s * P.batch) * elem;
+}
+
+bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  if (!a || !b) return false;
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
+
 // Launches passes [first, last) of the plan on `st`, its scratch leased from
-// the device's pool.
+// the device's pool.  Buffers may overlap: complex transforms run exactly in
+// place (src == dst, every pass reads a tile completely before it writes
+// the same tile, and the four-step passes go through scratch); any other
+// overlap of input and output (real transforms in place, shifted buffers)
+// first copies the input into the lease and runs from there.
 template <typename T>
 bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
              int last = -1) {
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
-  ScratchLease lease(P.device, scratch_total(P), st);
+  const bool il = P.format == afft_interleaved;
+  size_t in_b = 0, out_b = 0;
+  io_bytes(P, sizeof(T), in_b, out_b);
+  const void* ins[2] = {sre, (il || P.kind == 2) ? nullptr : sim};
+  const void* outs[2] = {dre, (il || P.kind == 3) ? nullptr : dim};
+  bool ov = false;
+  for (const void* a : ins)
+    for (const void* b : outs) ov = ov || overlaps(a, in_b, b, out_b);
+  const bool exact = P.kind <= 1 && sre == dre && (il || sim == dim);
+  const bool stage = ov && !exact;
+  const size_t base = scratch_total(P);
+  const size_t stage_off = (base + 255) & ~(size_t)255;
+  const size_t stage_b = (in_b + 255) & ~(size_t)255;
+  const int nstage = ins[1] ? 2 : 1;
+  ScratchLease lease(P.device, stage ? stage_off + nstage * stage_b : base, st);
   if (!lease.ok()) return false;
+  if (stage) {
+    char* sp = (char*)lease.ptr() + stage_off;
+    HIP_OK(hipMemcpyAsync(sp, sre, in_b, hipMemcpyDeviceToDevice, st));
+    sre = sp;
+    if (ins[1]) {
+      HIP_OK(hipMemcpyAsync(sp + stage_b, sim, in_b, hipMemcpyDeviceToDevice, st));
+      sim = sp + stage_b;
+    }
+  }
   return execute_on<T>(P, sre, sim, dre, dim, st, first, last, lease.ptr());
 }
 
 // Scratch bytes one execute needs (taken from the pool when it runs).
+// Bytes of the fused launch's scratch ring (counters follow it).
+size_t fused_ring_bytes(const Plan& P) {
+  return ((size_t)kFusedXcd * P.fused_slots * (size_t)P.dims[0] * sizeof(cx<float>) + 255) & ~(size_t)255;
+}
+
 void size_scratch(Plan& P, size_t elem_bytes) {
   const int pipe = (P.dual && P.batch > P.chunk) ? 1 : 0;
   P.pipe = pipe;
   for (int i = 0; i < 2; i++)
     P.scr_bytes[i] = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes * (i == 0 && pipe ? 2 : 1);
+  if (P.fused_slots)  // the ring and its counters in scratch 0 (pass ranges still run unfused)
+    P.scr_bytes[0] = std::max(P.scr_bytes[0], fused_ring_bytes(P) + sizeof(int) * fused_counter_ints(P.batch));
 }
 
 idx_t default_chunk(const Plan& P, size_t elem_bytes) {
@@ -1811,7 +1932,7 @@ TuneCache& tcache() {
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
 //   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline> [<real_pair>
-//   [<a_split>]]]]
+//   [<a_split> [<row_log2> [<fused> [<fused_slots> [<fused_lag>]]]]]]]]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -1823,9 +1944,9 @@ void tcache_load_locked(TuneCache& c) {
   while (fgets(line, sizeof(line), f)) {
     Variant v;
     long long ch = 0;
-    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d %d", key, &v.col_tile, &v.rowt_tile,
-                           &v.stream_io, &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline, &v.real_pair,
-                           &v.a_split);
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d", key, &v.col_tile,
+                           &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline,
+                           &v.real_pair, &v.a_split, &v.row_log2, &v.fused, &v.fused_slots, &v.fused_lag);
     if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
@@ -1852,8 +1973,9 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile, v.stream_io,
-          v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair, v.a_split);
+  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile,
+          v.stream_io, v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair, v.a_split,
+          v.row_log2, v.fused, v.fused_slots, v.fused_lag);
   fclose(f);
 }
 
@@ -2132,6 +2254,9 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.real_pair = v->real_pair;
   P.var.a_split = v->a_split;
   P.var.row_log2 = v->row_log2;
+  P.var.fused = v->fused;
+  P.var.fused_slots = v->fused_slots;
+  P.var.fused_lag = v->fused_lag;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -2184,6 +2309,9 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->real_pair = P.var.real_pair;
   v->a_split = P.var.a_split;
   v->row_log2 = P.var.row_log2;
+  v->fused = P.var.fused;
+  v->fused_slots = P.var.fused_slots;
+  v->fused_lag = P.var.fused_lag;
   return 0;
 }
 
@@ -2220,7 +2348,8 @@ int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t l
   if (r.type == PT_FFT)
     snprintf(buf, len,
              "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]") : (P.pipe && i < 2 ? "[fused]" : ""),
+             r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]")
+                    : (P.fused_slots ? "[fused4]" : (P.pipe && i < 2 ? "[fused]" : "")),
              r.L, r.CT, r.sv ? " sv1" : "",
              r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
              r.store_t ? " store_t" : "", (long long)r.n_col,
@@ -2297,6 +2426,9 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
     P->var.real_pair = v->real_pair;
     P->var.a_split = v->a_split;
     P->var.row_log2 = v->row_log2;
+    P->var.fused = v->fused;
+    P->var.fused_slots = v->fused_slots;
+    P->var.fused_lag = v->fused_lag;
   }
   g_dry = true;
   bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)

@@ -131,6 +131,13 @@ typedef struct
                         n = n1 * n2 (pass A: n2 columns of n1 points, pass B: n1 rows
                         of n2 points); 0 (default): 1024 up to 2^20, longer rows
                         beyond */
+  int32_t fused;     /* batched 1-D FP32 c2c four-step plans: 1 runs both passes in one
+                        persistent launch (pass-A and pass-B tiles of each transform on
+                        one XCD, the intermediate in a ring of scratch slots that stays
+                        in the MALL); -1 or 0: one launch per pass */
+  int32_t fused_slots; /* ring slots per XCD of the fused launch (0: 3) */
+  int32_t fused_lag;   /* rounds between a transform's pass A and its pass B in the
+                          fused launch (0: 1; < fused_slots) */
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);

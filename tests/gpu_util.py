@@ -133,3 +133,68 @@ def device_rel_l2(y_dev, ref_dev) -> float:
     torch.cuda.synchronize()
     a = acc.cpu().numpy()
     return float(np.sqrt(a[0] / a[1])) if a[1] > 0 else float("inf")
+
+
+def run_abi(kind: str, shape, x: np.ndarray, fmt=IL, dtype=np.float32, off: int = 0):
+    """One afft_XX_<kind>_create / _transform through the C ABI with every
+    buffer starting `off` elements past its allocation (afft's harness shifts
+    its buffers the same way, testing.cpp:67-90).  x is the input in afft's
+    layout for the kind (complex for c / ci / ri, real for r); returns the
+    output (complex, or real for ri) on the host."""
+    import ctypes
+    torch = torch_cuda()
+    L = _lib.lib()
+    p = "32" if np.dtype(dtype) == np.float32 else "64"
+    nd, dims = _lib.dims_array(shape)
+    mem = ctypes.create_string_buffer(getattr(L, f"afft_{p}_{kind}_memsize")(nd, dims, fmt, 0))
+    st = getattr(L, f"afft_{p}_{kind}_create")(nd, dims, fmt, mem, 0)
+    assert st, _lib.last_error()
+    n = int(np.prod(shape))
+    m = int(np.prod(half(shape)))
+    es = np.dtype(dtype).itemsize
+    cnt_in = {"c": n, "ci": n, "r": n, "ri": m}[kind]
+    cnt_out = {"c": n, "ci": n, "r": m, "ri": n}[kind]
+    cplx_in, cplx_out = kind != "r", kind != "ri"
+
+    def dev(cnt):
+        return empty_dev(cnt + off + 16, dtype)
+
+    def at(t):
+        return t.data_ptr() + off * es
+    xs = np.asarray(x).ravel()
+    if cplx_in and fmt == IL:
+        a = dev(2 * cnt_in)
+        a[off:off + 2 * cnt_in] = to_dev(_il(xs, dtype))
+        ins = (at(a), None)
+    elif cplx_in:
+        a, b = dev(cnt_in), dev(cnt_in)
+        a[off:off + cnt_in] = to_dev(np.ascontiguousarray(xs.real, dtype=dtype))
+        b[off:off + cnt_in] = to_dev(np.ascontiguousarray(xs.imag, dtype=dtype))
+        ins = (at(a), at(b))
+    else:
+        a = dev(cnt_in)
+        a[off:off + cnt_in] = to_dev(np.ascontiguousarray(xs, dtype=dtype))
+        ins = (at(a),)
+    if cplx_out and fmt == IL:
+        o = dev(2 * cnt_out)
+        outs, keep = (at(o), None), (o,)
+    elif cplx_out:
+        o, q = dev(cnt_out), dev(cnt_out)
+        outs, keep = (at(o), at(q)), (o, q)
+    else:
+        o = dev(cnt_out)
+        outs, keep = (at(o),), (o,)
+    fn = getattr(L, f"afft_{p}_{kind}_transform")
+    args = {"c": ins + outs, "ci": ins + outs, "r": ins + outs, "ri": (ins + (None,))[:2] + outs}[kind]
+    fn(st, *args)
+    err = _lib.last_error()
+    assert err == "", err
+    torch.cuda.synchronize()
+    h = [to_host(t)[off:] for t in keep]
+    if cplx_out and fmt == IL:
+        y = h[0][0:2 * cnt_out:2] + 1j * h[0][1:2 * cnt_out:2]
+    elif cplx_out:
+        y = h[0][:cnt_out] + 1j * h[1][:cnt_out]
+    else:
+        y = h[0][:cnt_out]
+    return y.reshape(half(shape) if kind == "r" else [int(s) for s in shape])
