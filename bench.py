@@ -6,10 +6,12 @@ inputs uniform[-1, 1) already resident in HBM.  value = GFLOP/s with the
 5*N*log2(N) convention per transform and direction.
 
 The JSON line also carries
-  roofline      the dominant kernel (four-step pass A, the 1024-point column
-                pass with the W_N twiddle fused), achieved algorithmic GB/s
-                against the 8 TB/s HBM peak, from HIP events on the stream the
-                kernel runs on;
+  roofline      the dominant kernel (the fused four-step launch, or the pass
+                of the two-pass plan that takes longest), achieved algorithmic
+                GB/s against the 8 TB/s HBM peak, from HIP events on the
+                stream the kernel runs on, timed inside steps; beside it the
+                step's own fraction and, when profiles/ holds a rocprofv3
+                kernel-trace summary of this command, the profiler's average;
   cpu_baseline  afft itself (oracle/_ref, built from the reference sources) on
                 this host's cores over a bounded sample of the same workload;
   extra         the rest of the afft configs measured the same way (r2c
@@ -74,6 +76,31 @@ def time_median(torch, stream, fn, iters, reps=5):
     return sorted(time_events(torch, stream, fn, iters) for _ in range(reps))[reps // 2]
 
 
+def cpu_info():
+    """(model name, logical CPUs of the host, CPUs this process may run on,
+    threads the baseline uses).  The thread count is the affinity set, capped
+    by OMP_NUM_THREADS when set (a shared host's CPU share for this job)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = avail
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    threads = int(os.environ.get("AFFT_BENCH_CPU_THREADS", threads))
+    return model, os.cpu_count() or 1, avail, max(1, threads)
+
+
 def cpu_baseline(seconds: float):
     """afft (reference build) on the host: 2^20 c2c forward + inverse,
     interleaved, one plan per thread, as many transforms as fit in about
@@ -83,7 +110,7 @@ def cpu_baseline(seconds: float):
         return None
     R = O.RefAfft()
     n = 1 << 20
-    threads = int(os.environ.get("AFFT_BENCH_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    model, nproc, avail, threads = cpu_info()
     x = O.complex_input(1, (n,))
     res = {}
     for nthr in sorted({1, threads}):
@@ -115,7 +142,7 @@ def cpu_baseline(seconds: float):
         total = sum(counts)
         res[nthr] = (flops_c2c(n, total) / el / 1e9, total, el)
     isa = "avx512f" if R.supported(O.AFFT_AVX512F) else "avx2" if R.supported(O.AFFT_AVX2) else "scalar"
-    return res, threads, isa
+    return res, threads, isa, model, nproc, avail
 
 
 def measured_traffic(describe: str, launch_transforms: int):
@@ -132,6 +159,22 @@ def measured_traffic(describe: str, launch_transforms: int):
         if js.get("describe") == describe and int(js.get("launch_transforms", -1)) == int(launch_transforms):
             return int(js["traffic_bytes_per_launch"]), os.path.relpath(f, ROOT)
     return None, None
+
+
+def rocprof_summary(describe: str):
+    """Average duration of the dominant kernel from the committed rocprofv3
+    kernel-trace summary of this bench command (profiles/*_ktrace.json,
+    written by fft_amd.profile_summary), if one matches the kernel."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_ktrace.json"))):
+        try:
+            with open(f) as fh:
+                js = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        if js.get("describe") == describe and "avg_ns" in js:
+            return {"avg_us": round(js["avg_ns"] / 1e3, 2), "source": os.path.relpath(f, ROOT)}
+    return None
 
 
 def bench_mfma(torch, dev, stream, g, iters=5):
@@ -280,25 +323,55 @@ def main():
     gflops = flops_c2c(n, 2 * batch) / (ms * 1e-3) / 1e9
     hbm_frac_step = 2 * batch * 16.0 * n / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS
 
-    # dominant kernel: pass A of the forward plan, timed on its own
-    launches = fwd.launches(0)
-    ms_a = time_events(torch, stream, lambda: fwd.run_range(0, 1, x, None, y, None, stream), args.steps)
-    ms_b = time_events(torch, stream, lambda: fwd.run_range(1, 2, x, None, y, None, stream), args.steps)
-    per_launch_ms = ms_a / launches
+    # the dominant kernel, timed inside steps: the same step sequence with
+    # HIP events on `stream` around every pass range (a fused plan runs its
+    # passes as one launch, so its range is the whole transform)
+    def ranges(p):
+        return [(0, p.num_passes)] if p.describe()[0].startswith("fft[fused4]") else \
+            [(i, i + 1) for i in range(p.num_passes)]
+    spans = [(p, a, b, src, dst) for p, src, dst in ((fwd, x, y), (inv, y, z)) for a, b in ranges(p)]
+    tot = [0.0] * len(spans)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(spans) + 1)]
+        evs[0].record(stream)
+        for k, (p, a, b, src, dst) in enumerate(spans):
+            p.run_range(a, b, src, None, dst, None, stream)
+            evs[k + 1].record(stream)
+        torch.cuda.synchronize()
+        for k in range(len(spans)):
+            tot[k] += evs[k].elapsed_time(evs[k + 1])
+    per_range = [t / args.steps for t in tot]
+    # pass ranges of the two directions share their kernel shapes: pick the
+    # range of the forward plan whose forward + inverse time is largest
+    nf = len(ranges(fwd))
+    comb = [per_range[k] + per_range[nf + k] for k in range(nf)]
+    kdom = int(np.argmax(comb))
+    a0, b0 = ranges(fwd)[kdom]
+    fused = b0 - a0 > 1
+    launches = 1 if fused else fwd.launches(a0)
+    per_launch_ms = comb[kdom] / 2 / launches
     bytes_per_launch = 16.0 * n * (batch / launches)
     achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    desc_a = fwd.describe()[0]
+    desc_a = fwd.describe()[a0]
     traffic, traffic_src = measured_traffic(desc_a, batch // launches)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": "four-step pass A: " + desc_a,
+                "kernel": ("fused four-step (both passes, one launch): " if fused else f"pass {a0}: ") + desc_a,
                 "avg_launch_us": round(per_launch_ms * 1e3, 2),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "step_frac": round(hbm_frac_step, 4),
                 "traffic_source": traffic_src}
+    rp = rocprof_summary(desc_a)
+    if rp:
+        roofline["rocprof_avg_us"] = rp["avg_us"]
+        roofline["frac_rocprof"] = round(bytes_per_launch / (rp["avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBPS, 4)
+        roofline["rocprof_source"] = rp["source"]
 
     extra = {"step_wall_ms": round(wall, 3), "step_hbm_frac_algorithmic": round(hbm_frac_step, 4),
-             "pass_a_ms_per_direction": round(ms_a, 4), "pass_b_ms_per_direction": round(ms_b, 4),
-             "chunk": int(fwd.get_variant().chunk), "pass_a_launch_transforms": int(batch / launches)}
+             "in_step_ms_per_range": {f"{'fwd' if k < nf else 'inv'} passes {spans[k][1]}-{spans[k][2] - 1}":
+                                      round(per_range[k], 4) for k in range(len(spans))},
+             "chunk": int(fwd.get_variant().chunk), "dominant_launch_transforms": int(batch / launches)}
     # measured copy ceiling on the same buffers
     ms_copy = time_events(torch, stream, lambda: _lib.lib().afft_gpu_copy(int(x.data_ptr()), int(z.data_ptr()),
                                                                          x.numel() * 4, stream.cuda_stream), 10)
@@ -366,14 +439,15 @@ def main():
     if not args.no_cpu:
         r = cpu_baseline(args.cpu_seconds)
         if r is not None:
-            res, threads, isa = r
+            res, threads, isa, model, nproc, avail = r
             v1 = res[1]
             vt = res[threads]
             cpu = {"value": round(vt[0], 2), "unit": "GFLOP/s", "cores": threads, "kind": "reference",
                    "sample": f"afft ({isa}, oracle/_ref) c2c N=2^20 interleaved fwd+inv, one plan per thread, "
                              f"{vt[1]} transforms in {vt[2]:.1f} s on {threads} threads; single thread: "
                              f"{v1[0]:.2f} GFLOP/s ({v1[1]} transforms in {v1[2]:.1f} s)",
-                   "single_thread_value": round(v1[0], 2)}
+                   "single_thread_value": round(v1[0], 2), "cpu_model": model, "isa": isa,
+                   "host_logical_cpus": nproc, "cpus_available": avail}
 
     out = {
         "metric": "batched c2c FFT GFLOP/s (5N*log2N), 64 x N=2^20 FP32 fwd+inv",

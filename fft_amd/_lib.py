@@ -35,7 +35,7 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_queue_run", "afft_gpu_queue_num_tasks", "afft_gpu_queue_destroy", "afft_gpu_queue_status",
     "afft_gpu_queue_set_spin_limit", "afft_gpu_queue_set_groups_per_cu",
     "afft_gpu_plan_set_mode", "afft_gpu_mfma_selftest", "afft_gpu_plan_set_variant", "afft_gpu_plan_get_variant",
-    "afft_gpu_plan_tune",
+    "afft_gpu_plan_tune", "afft_gpu_plan_describe_variant", "afft_gpu_execute_noalloc",
 ]
 
 
@@ -83,6 +83,8 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_plan_create.restype = vp
     L.afft_gpu_execute.argtypes = [vp, vp, vp, vp, vp, vp]
     L.afft_gpu_execute.restype = ctypes.c_int
+    L.afft_gpu_execute_noalloc.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.afft_gpu_execute_noalloc.restype = ctypes.c_int
     L.afft_gpu_execute_range.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp]
     L.afft_gpu_execute_range.restype = ctypes.c_int
     L.afft_gpu_plan_launches.argtypes = [vp, ctypes.c_int]
@@ -137,6 +139,9 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_plan_get_variant.restype = ctypes.c_int
     L.afft_gpu_plan_tune.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int, vp]
     L.afft_gpu_plan_tune.restype = ctypes.c_int
+    L.afft_gpu_plan_describe_variant.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, c_size_p, ctypes.c_int,
+                                                 ctypes.c_size_t, vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    L.afft_gpu_plan_describe_variant.restype = ctypes.c_int
     _LIB = L
     return L
 

@@ -77,8 +77,17 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"afft_run build failed:\n{r.stderr[-4000:]}")
     os.replace(harness + ".tmp", harness)
+    # the libbench2 adapter with its stand-alone driver (fft_amd/harness/bench2_afft.cpp)
+    bench2 = os.path.join(PKG, "afft_bench2")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-result", "-I", os.path.join(PKG, "..", "include"),
+           os.path.join(PKG, "harness", "bench2_afft.cpp"), "-L", PKG, "-lafft_mi355x", "-Wl,-rpath,$ORIGIN",
+           "-o", bench2 + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"afft_bench2 build failed:\n{r.stderr[-4000:]}")
+    os.replace(bench2 + ".tmp", bench2)
     if verbose:
-        print("built", LIB, "and", harness)
+        print("built", LIB, "and", harness, "and", bench2)
     return LIB
 
 

@@ -38,16 +38,6 @@ __device__ __forceinline__ int fused_xcc_id() {
   return (int)(v & (kFusedXcd - 1));
 }
 
-// The tables' addresses are loop-invariant, and so is every per-thread
-// twiddle read from them: laundering the base address once per task keeps
-// the compiler from hoisting those reads out of the task loop and holding
-// them in registers across tasks.
-template <typename P>
-__device__ __forceinline__ P per_task(P p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
-
 // One tile of each pass, as functions of their own: inlined into the task
 // loop, the compiler keeps values of one task live across the next and
 // spills; called, each body gets the register allocation of its standalone
@@ -55,7 +45,11 @@ __device__ __forceinline__ P per_task(P p) {
 // first (buffer resources need scalar registers).
 template <typename P>
 __device__ __forceinline__ P uniform(P p) {
-  return (P)__builtin_amdgcn_readfirstlane((unsigned long long)p);
+  // readfirstlane moves 32 bits: a 64-bit address takes two
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (P)(((unsigned long long)hi << 32) | lo);
 }
 
 template <int LA, int CTA, int SVA, bool HXA, int LB>

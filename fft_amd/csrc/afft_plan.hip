@@ -116,6 +116,11 @@ struct PassRec {
   const void* mfma_mats = nullptr;
 };
 
+// The fused four-step launch (afft_fused.hip) stays off: its first GPU run
+// after a pointer fix did not finish within its time limit, and no cause has
+// been shown yet.  A variant asking for it is rejected.
+constexpr bool kFusedEnabled = false;
+
 // Strided (column) axes up to this length run as one column pass; longer
 // ones as a two-pass column four-step with balanced factors, whose tiles span
 // more contiguous columns than one long column tile can.
@@ -958,6 +963,12 @@ struct Builder {
   // variant asks for it.
   void pick_fused() {
     P.fused_slots = P.fused_lag = 0;
+    if (P.var.fused > 0 && !kFusedEnabled) {
+      set_err("the fused four-step launch is switched off: its first run did not finish (see the fused kernel's "
+              "notes); use fused = 0");
+      ok = false;
+      return;
+    }
     if (P.var.fused <= 0 || sizeof(T) != 4 || P.npass != 2 || P.dual || P.mode != afft_gpu_mode_fma ||
         P.kind > 1 || P.ndim != 1 || P.format != afft_interleaved)
       return;
@@ -1596,6 +1607,11 @@ int a_split(const Plan& P) {
   return (int)(P.chunk / g);
 }
 
+// Bytes of the fused launch's scratch ring (counters follow it).
+size_t fused_ring_bytes(const Plan& P) {
+  return ((size_t)kFusedXcd * P.fused_slots * (size_t)P.dims[0] * sizeof(cx<float>) + 255) & ~(size_t)255;
+}
+
 // Scratch 1 starts at this offset into the plan's scratch (scratch 0 first).
 size_t scratch1_offset(const Plan& P) { return (P.scr_bytes[0] + 255) & ~(size_t)255; }
 size_t scratch_total(const Plan& P) { return P.scr_bytes[1] ? scratch1_offset(P) + P.scr_bytes[1] : P.scr_bytes[0]; }
@@ -1604,15 +1620,22 @@ size_t scratch_total(const Plan& P) { return P.scr_bytes[1] ? scratch1_offset(P)
 // (scratch_total(P) bytes).
 template <typename T>
 bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first, int last,
-                void* scr) {
+                void* scr, idx_t chunk = 0, bool plain = false) {
+  // chunk > 0: groups of that many transforms (scratch sized for it) and,
+  // with plain, one launch per pass only (no fused or pipelined schedule)
   if (last < 0) last = P.npass;
+  if (chunk <= 0) chunk = P.chunk;
+  // scratch 1 follows scratch 0 of the group (the pool layout also makes
+  // room for the fused ring or the pipelined halves, which plain runs skip)
+  const size_t scr1_off = !plain ? scratch1_offset(P)
+                                 : ((size_t)P.scr_per_user[0] * (size_t)chunk * sizeof(cx<T>) + 255) & ~(size_t)255;
   Bufs<T> B{(const T*)sre, (const T*)sim, (T*)dre, (T*)dim,
-            {(T*)scr, P.scr_bytes[1] ? (T*)((char*)scr + scratch1_offset(P)) : nullptr}};
+            {(T*)scr, P.scr_bytes[1] ? (T*)((char*)scr + scr1_off) : nullptr}};
   if (P.format == afft_interleaved) {
     B.src_im = nullptr;
     B.dst_im = nullptr;
   }
-  if (P.fused_slots && first == 0 && last == P.npass) {
+  if (P.fused_slots && !plain && first == 0 && last == P.npass) {
     // both passes in one persistent launch over the whole batch
     const PassRec& a = P.pass[0];
     const PassRec& b = P.pass[1];
@@ -1638,7 +1661,7 @@ bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim,
     HIP_OK(fused_launch(f, a.L, a.CT, a.sv, a.hx, b.L, b.CT, b.sv, b.hx, st));
     return true;
   }
-  if (P.pipe && first == 0 && last == P.npass) {
+  if (P.pipe && !plain && first == 0 && last == P.npass) {
     // launch k: pass A of group k into scratch half k%2 beside pass B of
     // group k-1 from half (k-1)%2; the launches are ordered on `st`
     const DualVariant<T>* d = (const DualVariant<T>*)P.dual;
@@ -1674,8 +1697,8 @@ bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim,
     return true;
   }
   const int split = a_split(P);
-  for (idx_t start = 0; start < P.batch; start += P.chunk) {
-    idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
+  for (idx_t start = 0; start < P.batch; start += chunk) {
+    idx_t cnt = P.batch - start < chunk ? P.batch - start : chunk;
     for (int i = first; i < last; i++) {
       if (i == 0 && split > 1 && cnt % split == 0) {
         // pass A of the group in `split` launches, each into its part of the
@@ -1721,17 +1744,20 @@ bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
   return x < y + nb && y < x + na;
 }
 
-// Launches passes [first, last) of the plan on `st`, its scratch leased from
-// the device's pool.  Buffers may overlap: complex transforms run exactly in
-// place (src == dst, every pass reads a tile completely before it writes
-// the same tile, and the four-step passes go through scratch); any other
-// overlap of input and output (real transforms in place, shifted buffers)
-// first copies the input into the lease and runs from there.
+// Buffers may overlap: complex transforms run exactly in place (src == dst:
+// every pass reads a tile completely before it writes the same tile, and the
+// four-step passes go through scratch); any other overlap of input and output
+// (real transforms in place, shifted buffers) first copies the input into
+// scratch and runs from there.  Staging says how many input arrays to copy
+// (0 = none) and their size.
+struct Staging {
+  int n = 0;          // arrays to copy (re; im too for split complex input)
+  size_t bytes = 0;   // bytes of one array
+  size_t stride = 0;  // bytes between the copies in scratch
+};
+
 template <typename T>
-bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
-             int last = -1) {
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+Staging staging(const Plan& P, const void* sre, const void* sim, const void* dre, const void* dim) {
   const bool il = P.format == afft_interleaved;
   size_t in_b = 0, out_b = 0;
   io_bytes(P, sizeof(T), in_b, out_b);
@@ -1741,31 +1767,44 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
   for (const void* a : ins)
     for (const void* b : outs) ov = ov || overlaps(a, in_b, b, out_b);
   const bool exact = P.kind <= 1 && sre == dre && (il || sim == dim);
-  const bool stage = ov && !exact;
+  Staging sg;
+  if (ov && !exact) {
+    sg.n = ins[1] ? 2 : 1;
+    sg.bytes = in_b;
+    sg.stride = (in_b + 255) & ~(size_t)255;
+  }
+  return sg;
+}
+
+// Copies the input into scratch at `at` (sg.n arrays) and points sre / sim there.
+bool stage_input(const Staging& sg, char* at, const void*& sre, const void*& sim, hipStream_t st) {
+  if (sg.n == 0) return true;
+  HIP_OK(hipMemcpyAsync(at, sre, sg.bytes, hipMemcpyDeviceToDevice, st));
+  sre = at;
+  if (sg.n == 2) {
+    HIP_OK(hipMemcpyAsync(at + sg.stride, sim, sg.bytes, hipMemcpyDeviceToDevice, st));
+    sim = at + sg.stride;
+  }
+  return true;
+}
+
+// Launches passes [first, last) of the plan on `st`, its scratch (and room
+// for a staged input) leased from the device's pool.
+template <typename T>
+bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first = 0,
+             int last = -1) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+  const Staging sg = staging<T>(P, sre, sim, dre, dim);
   const size_t base = scratch_total(P);
   const size_t stage_off = (base + 255) & ~(size_t)255;
-  const size_t stage_b = (in_b + 255) & ~(size_t)255;
-  const int nstage = ins[1] ? 2 : 1;
-  ScratchLease lease(P.device, stage ? stage_off + nstage * stage_b : base, st);
+  ScratchLease lease(P.device, sg.n ? stage_off + sg.n * sg.stride : base, st);
   if (!lease.ok()) return false;
-  if (stage) {
-    char* sp = (char*)lease.ptr() + stage_off;
-    HIP_OK(hipMemcpyAsync(sp, sre, in_b, hipMemcpyDeviceToDevice, st));
-    sre = sp;
-    if (ins[1]) {
-      HIP_OK(hipMemcpyAsync(sp + stage_b, sim, in_b, hipMemcpyDeviceToDevice, st));
-      sim = sp + stage_b;
-    }
-  }
+  if (!stage_input(sg, (char*)lease.ptr() + stage_off, sre, sim, st)) return false;
   return execute_on<T>(P, sre, sim, dre, dim, st, first, last, lease.ptr());
 }
 
 // Scratch bytes one execute needs (taken from the pool when it runs).
-// Bytes of the fused launch's scratch ring (counters follow it).
-size_t fused_ring_bytes(const Plan& P) {
-  return ((size_t)kFusedXcd * P.fused_slots * (size_t)P.dims[0] * sizeof(cx<float>) + 255) & ~(size_t)255;
-}
-
 void size_scratch(Plan& P, size_t elem_bytes) {
   const int pipe = (P.dual && P.batch > P.chunk) ? 1 : 0;
   P.pipe = pipe;
@@ -2112,6 +2151,157 @@ int stage_list(int L, int sv, int* radix, int* ns) {
 }
 }  // namespace afgpu
 
+namespace {
+// One line describing pass i of a plan (kernel shape and buffer roles).
+int describe_pass(const Plan& P, int i, char* buf, size_t len) {
+  if (i < 0 || i >= P.npass || !buf) return -1;
+  const PassRec& r = P.pass[i];
+  static const char* ep[] = {"plain", "tw4", "r2c", "c2r", "c2r+tw4"};
+  static_assert(sizeof(ep) / sizeof(ep[0]) == EPI_C2R4 + 1, "one name per epilogue");
+  if (r.type == PT_FFT)
+    snprintf(buf, len,
+             "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
+             r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]")
+                    : (P.fused_slots ? "[fused4]" : (P.pipe && i < 2 ? "[fused]" : "")),
+             r.L, r.CT, r.sv ? " sv1" : "",
+             r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
+             r.store_t ? " store_t" : "", (long long)r.n_col,
+             (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
+  else if (r.type == PT_REALTR)
+    snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
+  else if (r.type == PT_REALIROWFFT)
+    snprintf(buf, len, "real_irowfft L=%d h=%lld in(role=%d il=%d)", r.L, (long long)r.rr_h, r.in.role, r.in.il);
+  else if (r.type == PT_REALROWFFT)
+    snprintf(buf, len, "real_rowfft L=%d h=%lld out(role=%d il=%d)", r.L, (long long)r.rr_h, r.out.role, r.out.il);
+  else if (r.type == PT_REALPAIR)
+    snprintf(buf, len, "real_pair L=%d rows=%d h=%lld n1=%lld out(role=%d il=%d)", r.L, r.CT, (long long)r.rr_h,
+             (long long)r.rr_m, r.out.role, r.out.il);
+  else if (r.type == PT_REALROWS)
+    snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
+  else if (r.type == PT_REALICOLPAIR)
+    snprintf(buf, len, "real_icolpair L=%d CT=%d h2=%d h=%lld m=%lld in(role=%d il=%d)", r.L, r.CT, r.n_blo,
+             (long long)r.rr_h, (long long)r.rr_m, r.in.role, r.in.il);
+  else if (r.type == PT_REALCOLPAIR)
+    snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
+             (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);
+  else
+    snprintf(buf, len, "real_one inv=%d", r.rr_inv);
+  return 0;
+}
+
+// The plan a shape and variant build, on the host only (no device calls;
+// table addresses are placeholders).  nullptr on error.
+Plan* dry_plan(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+               const afft_gpu_variant* v) {
+  idx_t d[kMaxDim];
+  int nd = 0;
+  if (validate(ndim, dim, format, afft_gpu, d, nd) != 0) return nullptr;
+  if (batch < 1 || kind < 0 || kind > 3 || (precision != 32 && precision != 64)) {
+    set_err("invalid arguments");
+    return nullptr;
+  }
+  Plan* P = new (std::nothrow) Plan();
+  if (!P) return nullptr;
+  P->prec = precision;
+  P->kind = kind;
+  P->format = format;
+  P->ndim = nd;
+  for (int i = 0; i < nd; i++) P->dims[i] = d[i];
+  P->batch = (idx_t)batch;
+  if (v) {
+    P->var.col_tile = v->col_tile;
+    P->var.rowt_tile = v->rowt_tile;
+    P->var.stream_io = v->stream_io;
+    P->var.xcd_order = v->xcd_order;
+    P->var.chunk = v->chunk;
+    P->var.sched = v->sched;
+    P->var.half_lds = v->half_lds;
+    P->var.pipeline = v->pipeline;
+    P->var.real_pair = v->real_pair;
+    P->var.a_split = v->a_split;
+    P->var.row_log2 = v->row_log2;
+    P->var.fused = v->fused;
+    P->var.fused_slots = v->fused_slots;
+    P->var.fused_lag = v->fused_lag;
+  }
+  g_dry = true;
+  bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)
+                            : Builder<double>(*P).build(kind, format, d, nd);
+  g_dry = false;
+  if (!ok) {
+    delete P;
+    return nullptr;
+  }
+  const size_t eb = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
+  P->chunk = default_chunk(*P, eb);
+  size_scratch(*P, eb);
+  return P;
+}
+
+// ---- execution with no allocation ---------------------------------------------
+// afft_gpu_execute_noalloc runs a plan on a fixed scratch buffer compiled into
+// the library (kNoAllocBytes of device memory per GPU, held from the library's
+// load to the end of the process) instead of the pool: no allocation, so
+// callers that must not allocate (code under graph capture that may not grow
+// the pool, tight serving loops) can use it.  The array's name is a host-side
+// stand-in; its device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+void* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_err("device %d out of range", dev);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_err("no-allocation scratch: hipGetSymbolAddress failed: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_noalloc_ptr[dev] = ptr;
+  }
+  return g_noalloc_ptr[dev];
+}
+
+// Runs the plan with its scratch (and a staged input) in the fixed buffer:
+// transforms are grouped as the plan groups them, in smaller groups if that
+// is what fits, and always one launch per pass.
+template <typename T>
+bool execute_noalloc(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev != P.device) HIP_OK(hipSetDevice(P.device));
+  void* base = noalloc_scratch(P.device);
+  if (!base) return false;
+  const Staging sg = staging<T>(P, sre, sim, dre, dim);
+  const size_t stage_need = sg.n * sg.stride;
+  auto need = [&](idx_t c) -> size_t {  // scratch of a group of c transforms, then the staged input
+    const size_t s0 = (size_t)P.scr_per_user[0] * (size_t)c * sizeof(cx<T>);
+    const size_t s1 = (size_t)P.scr_per_user[1] * (size_t)c * sizeof(cx<T>);
+    const size_t tot = s1 ? ((s0 + 255) & ~(size_t)255) + s1 : s0;
+    return ((tot + 255) & ~(size_t)255) + stage_need;
+  };
+  idx_t c = P.chunk;
+  while (c > 1 && need(c) > kNoAllocBytes) c = (c + 1) / 2;
+  if (need(c) > kNoAllocBytes) {
+    set_err("the plan needs %zu bytes of scratch for one transform, above the %zu bytes of the no-allocation "
+            "buffer",
+            need(1), kNoAllocBytes);
+    return false;
+  }
+  const size_t stage_off = need(c) - stage_need;
+  if (!stage_input(sg, (char*)base + stage_off, sre, sim, st)) return false;
+  return execute_on<T>(P, sre, sim, dre, dim, st, 0, P.npass, base, c, true);
+}
+
+}  // namespace
+
 // ---- C ABI ----------------------------------------------------------------------
 extern "C" {
 
@@ -2186,6 +2376,16 @@ afft_gpu_plan* afft_gpu_plan_create(int precision, int kind, size_t ndim, const 
 
 int afft_gpu_execute(afft_gpu_plan* plan, const void* sre, const void* sim, void* dre, void* dim, void* stream) {
   return afft_gpu_execute_range(plan, 0, -1, sre, sim, dre, dim, stream);
+}
+
+int afft_gpu_execute_noalloc(afft_gpu_plan* plan, const void* sre, const void* sim, void* dre, void* dim,
+                             void* stream) {
+  g_err.clear();
+  Plan* P = get_plan(plan);
+  if (!P) return -1;
+  bool ok = P->prec == 32 ? execute_noalloc<float>(*P, sre, sim, dre, dim, (hipStream_t)stream)
+                          : execute_noalloc<double>(*P, sre, sim, dre, dim, (hipStream_t)stream);
+  return ok ? 0 : -1;
 }
 
 int afft_gpu_execute_range(afft_gpu_plan* plan, int first, int last, const void* sre, const void* sim, void* dre,
@@ -2338,42 +2538,9 @@ size_t afft_gpu_plan_scratch_bytes(const afft_gpu_plan* plan) {
 }
 
 int afft_gpu_plan_describe(const afft_gpu_plan* plan, int i, char* buf, size_t len) {
-  Plan* Pp = get_plan(plan);
-  if (!Pp) return -1;
-  const Plan& P = *Pp;
-  if (i < 0 || i >= P.npass || !buf) return -1;
-  const PassRec& r = P.pass[i];
-  static const char* ep[] = {"plain", "tw4", "r2c", "c2r", "c2r+tw4"};
-  static_assert(sizeof(ep) / sizeof(ep[0]) == EPI_C2R4 + 1, "one name per epilogue");
-  if (r.type == PT_FFT)
-    snprintf(buf, len,
-             "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
-             r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]")
-                    : (P.fused_slots ? "[fused4]" : (P.pipe && i < 2 ? "[fused]" : "")),
-             r.L, r.CT, r.sv ? " sv1" : "",
-             r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
-             r.store_t ? " store_t" : "", (long long)r.n_col,
-             (long long)r.b_per_user, r.n_blo, r.in.role, r.in.il, r.out.role, r.out.il);
-  else if (r.type == PT_REALTR)
-    snprintf(buf, len, "real_tr h=%lld n1=%lld", (long long)r.rr_h, (long long)r.rr_m);
-  else if (r.type == PT_REALIROWFFT)
-    snprintf(buf, len, "real_irowfft L=%d h=%lld in(role=%d il=%d)", r.L, (long long)r.rr_h, r.in.role, r.in.il);
-  else if (r.type == PT_REALROWFFT)
-    snprintf(buf, len, "real_rowfft L=%d h=%lld out(role=%d il=%d)", r.L, (long long)r.rr_h, r.out.role, r.out.il);
-  else if (r.type == PT_REALPAIR)
-    snprintf(buf, len, "real_pair L=%d rows=%d h=%lld n1=%lld out(role=%d il=%d)", r.L, r.CT, (long long)r.rr_h,
-             (long long)r.rr_m, r.out.role, r.out.il);
-  else if (r.type == PT_REALROWS)
-    snprintf(buf, len, "real_rows h=%lld m=%lld inv=%d", (long long)r.rr_h, (long long)r.rr_m, r.rr_inv);
-  else if (r.type == PT_REALICOLPAIR)
-    snprintf(buf, len, "real_icolpair L=%d CT=%d h2=%d h=%lld m=%lld in(role=%d il=%d)", r.L, r.CT, r.n_blo,
-             (long long)r.rr_h, (long long)r.rr_m, r.in.role, r.in.il);
-  else if (r.type == PT_REALCOLPAIR)
-    snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
-             (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);
-  else
-    snprintf(buf, len, "real_one inv=%d", r.rr_inv);
-  return 0;
+  Plan* P = get_plan(plan);
+  if (!P) return -1;
+  return describe_pass(*P, i, buf, len);
 }
 
 const char* afft_gpu_last_error(void) { return g_err.c_str(); }
@@ -2399,49 +2566,11 @@ int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* di
 int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
                                   const afft_gpu_variant* v, afft_gpu_plan_info* info, afft_gpu_pass_info* passes,
                                   int max_passes) {
-  idx_t d[kMaxDim];
-  int nd = 0;
-  if (validate(ndim, dim, format, afft_gpu, d, nd) != 0) return -1;
-  if (batch < 1 || kind < 0 || kind > 3 || (precision != 32 && precision != 64)) {
-    set_err("invalid arguments");
-    return -1;
-  }
-  Plan* P = new (std::nothrow) Plan();
+  Plan* P = dry_plan(precision, kind, ndim, dim, format, batch, v);
   if (!P) return -1;
-  P->prec = precision;
-  P->kind = kind;
-  P->format = format;
-  P->ndim = nd;
-  for (int i = 0; i < nd; i++) P->dims[i] = d[i];
-  P->batch = (idx_t)batch;
-  if (v) {
-    P->var.col_tile = v->col_tile;
-    P->var.rowt_tile = v->rowt_tile;
-    P->var.stream_io = v->stream_io;
-    P->var.xcd_order = v->xcd_order;
-    P->var.chunk = v->chunk;
-    P->var.sched = v->sched;
-    P->var.half_lds = v->half_lds;
-    P->var.pipeline = v->pipeline;
-    P->var.real_pair = v->real_pair;
-    P->var.a_split = v->a_split;
-    P->var.row_log2 = v->row_log2;
-    P->var.fused = v->fused;
-    P->var.fused_slots = v->fused_slots;
-    P->var.fused_lag = v->fused_lag;
-  }
-  g_dry = true;
-  bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)
-                            : Builder<double>(*P).build(kind, format, d, nd);
-  g_dry = false;
-  if (!ok) {
-    delete P;
-    return -1;
-  }
-  size_t eb = precision == 32 ? sizeof(cx<float>) : sizeof(cx<double>);
-  P->chunk = default_chunk(*P, eb);
   if (info) {
     info->npass = P->npass;
+    info->fused_slots = P->fused_slots;
     info->chunk = P->chunk;
     info->scr_per_user[0] = P->scr_per_user[0];
     info->scr_per_user[1] = P->scr_per_user[1];
@@ -2475,6 +2604,15 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
   int np = P->npass;
   delete P;
   return np;
+}
+
+int afft_gpu_plan_describe_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                                   const afft_gpu_variant* v, int i, char* buf, size_t len) {
+  Plan* P = dry_plan(precision, kind, ndim, dim, format, batch, v);
+  if (!P) return -1;
+  const int r = describe_pass(*P, i, buf, len);
+  delete P;
+  return r;
 }
 
 int afft_gpu_error_sums_f32(const float* y, const double* ref, size_t n, double* acc, void* stream) {

@@ -11,8 +11,13 @@
         the same counter read the known size C of a plain device copy (D3, D4:
         C bytes read and C bytes written per call).
 
+    python -m fft_amd.profile_summary ktrace <dir>/<name>_kernel_stats.csv --kernel REGEX
+        --describe "pass description" --command "..." --out profiles/<name>_ktrace.json
+        average duration of the kernel matching REGEX in a kernel-trace run
+
 The pmc summary is what bench.py reads to fill roofline.traffic when the
-dominant pass of the current plan matches `describe` and `launch_transforms`.
+dominant pass of the current plan matches `describe` and `launch_transforms`;
+the ktrace summary gives roofline.rocprof_avg_us for the same kernel.
 """
 from __future__ import annotations
 
@@ -56,6 +61,17 @@ def stats_table(path: str) -> str:
     return "\n".join(out)
 
 
+def ktrace_summary(a) -> dict:
+    with open(a.csv) as fh:
+        rows = [r for r in csv.DictReader(fh) if re.search(a.kernel, r["Name"])]
+    if len(rows) != 1:
+        raise ValueError(f"{len(rows)} kernels match /{a.kernel}/ in {a.csv}")
+    r = rows[0]
+    return {"kernel_regex": a.kernel, "kernel": r["Name"], "describe": a.describe, "calls": int(r["Calls"]),
+            "avg_ns": float(r["AverageNs"]), "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
+            "command": a.command}
+
+
 def pmc_summary(a) -> dict:
     kib = 1024.0
     fetch, nf = counter_mean(a.fetch, "FETCH_SIZE", a.kernel)
@@ -95,9 +111,18 @@ def main(argv=None):
     p.add_argument("--launch-transforms", type=int, required=True)
     p.add_argument("--algo-bytes", type=float, required=True)
     p.add_argument("--copy-bytes", type=float, required=True)
+    k = sub.add_parser("ktrace")
+    k.add_argument("csv")
+    for f in ("kernel", "describe", "command", "out"):
+        k.add_argument("--" + f, dest=f, required=True)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats_table(a.csv))
+    elif a.cmd == "ktrace":
+        js = ktrace_summary(a)
+        with open(a.out, "w") as fh:
+            json.dump(js, fh, indent=1)
+        print(json.dumps(js, indent=1))
     else:
         js = pmc_summary(a)
         with open(a.out, "w") as fh:

@@ -55,6 +55,24 @@ afft_gpu_plan* afft_gpu_plan_create(int precision, int kind, size_t ndim, const 
 int afft_gpu_execute(afft_gpu_plan* plan, const void* src_re, const void* src_im, void* dst_re, void* dst_im,
                      void* stream);
 
+/* The same transform as afft_gpu_execute without any allocation: no
+ * counterpart in the reference (afft allocates nothing at transform time
+ * because its plan memory holds its work arrays).  For callers that cannot
+ * allocate device memory: code under HIP graph capture that must not grow the
+ * scratch pool, serving loops and harnesses that forbid hipMalloc.  The plan's
+ * scratch (and the copy of an input that overlaps the output) comes from a
+ * fixed 64 MiB device buffer compiled into the library, one per GPU, held from
+ * the library's load until the process ends; transforms run in groups that
+ * fit it, one launch per pass (the fused and pipelined schedules are not
+ * used).  A plan whose single transform needs more than 64 MiB of scratch is
+ * rejected with -1 and a message (afft_gpu_last_error), and nothing runs.
+ * Every call on a GPU shares that GPU's buffer and it keeps the last call's
+ * contents (no kernel here reads scratch it did not write first): calls that
+ * overlap on one GPU -- on two streams, or from two threads -- race.
+ * Returns 0 on success, -1 on error. */
+int afft_gpu_execute_noalloc(afft_gpu_plan* plan, const void* src_re, const void* src_im, void* dst_re, void* dst_im,
+                             void* stream);
+
 /* Enqueues only passes [first, last) of the plan (last = -1: to the end),
  * for every chunk of the batch.  For measuring one kernel of a multi-pass
  * plan; running the ranges in order equals afft_gpu_execute. */
@@ -131,10 +149,12 @@ typedef struct
                         n = n1 * n2 (pass A: n2 columns of n1 points, pass B: n1 rows
                         of n2 points); 0 (default): 1024 up to 2^20, longer rows
                         beyond */
-  int32_t fused;     /* batched 1-D FP32 c2c four-step plans: 1 runs both passes in one
-                        persistent launch (pass-A and pass-B tiles of each transform on
-                        one XCD, the intermediate in a ring of scratch slots that stays
-                        in the MALL); -1 or 0: one launch per pass */
+  int32_t fused;     /* batched 1-D FP32 c2c four-step plans: 1 would run both passes in
+                        one persistent launch (pass-A and pass-B tiles of each transform
+                        on one XCD, the intermediate in a ring of scratch slots that
+                        stays in the MALL); that launch is switched off in this build
+                        and a variant with fused = 1 is rejected; -1 or 0: one launch
+                        per pass */
   int32_t fused_slots; /* ring slots per XCD of the fused launch (0: 3) */
   int32_t fused_lag;   /* rounds between a transform's pass A and its pass B in the
                           fused launch (0: 1; < fused_slots) */
@@ -266,7 +286,9 @@ typedef struct
 
 typedef struct
 {
-  int32_t npass, pad_;
+  int32_t npass;
+  int32_t fused_slots; /* > 0: both passes run in one persistent launch with this many
+                          ring slots per XCD (afft_gpu_variant.fused) */
   int64_t chunk;
   int64_t scr_per_user[2];
 } afft_gpu_plan_info;
@@ -282,6 +304,12 @@ int afft_gpu_plan_inspect(int precision, int kind, size_t ndim, const size_t* di
 int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
                                   const afft_gpu_variant* v, afft_gpu_plan_info* info, afft_gpu_pass_info* passes,
                                   int max_passes);
+
+/* The one-line description afft_gpu_plan_describe gives for pass i of the
+ * plan this shape and variant (NULL: default) build, built on the host only.
+ * Returns 0, or -1 on error or when i is out of range. */
+int afft_gpu_plan_describe_variant(int precision, int kind, size_t ndim, const size_t* dim, int format, size_t batch,
+                                   const afft_gpu_variant* v, int i, char* buf, size_t len);
 
 /* ---- helpers -------------------------------------------------------------- */
 /* acc[0] = sum (y[i] - ref[i])^2, acc[1] = sum ref[i]^2 over n values, in

@@ -37,7 +37,7 @@ class PassInfo(ctypes.Structure):
 
 
 class PlanInfo(ctypes.Structure):
-    _fields_ = [("npass", ctypes.c_int32), ("pad_", ctypes.c_int32), ("chunk", ctypes.c_int64),
+    _fields_ = [("npass", ctypes.c_int32), ("fused_slots", ctypes.c_int32), ("chunk", ctypes.c_int64),
                 ("scr_per_user", ctypes.c_int64 * 2)]
 
 
