@@ -1861,6 +1861,7 @@ int validate(size_t ndim, const size_t* dim, int format, int impl, idx_t* d, int
 }
 
 bool tcache_get(const Plan& P, Variant& v);  // defined with the autotuner below
+void* noalloc_scratch(int dev);              // defined with afft_gpu_execute_noalloc below
 
 // Bytes of `mem` a plan needs: the Plan itself at the first 64-byte boundary.
 constexpr size_t kPlanMem = sizeof(Plan) + kPlanAlign - 1;
@@ -1901,6 +1902,10 @@ void* create_impl(int kind, size_t ndim, const size_t* dim, int format, size_t b
   }
   P->chunk = default_chunk(*P, sizeof(cx<T>));
   size_scratch(*P, sizeof(cx<T>));
+  // set-up of the no-allocation entry point: resolve this device's fixed
+  // buffer now, so that its first call allocates nothing either (a failure
+  // here only matters to that entry point, which reports it)
+  if (!noalloc_scratch(P->device)) g_err.clear();
   P->magic = kMagic;
   return P;
 }
@@ -2240,12 +2245,13 @@ Plan* dry_plan(int precision, int kind, size_t ndim, const size_t* dim, int form
 
 // ---- execution with no allocation ---------------------------------------------
 // afft_gpu_execute_noalloc runs a plan on a fixed scratch buffer compiled into
-// the library (kNoAllocBytes of device memory per GPU, held from the library's
-// load to the end of the process) instead of the pool: no allocation, so
-// callers that must not allocate (code under graph capture that may not grow
-// the pool, tight serving loops) can use it.  The array's name is a host-side
-// stand-in; its device address comes from hipGetSymbolAddress, once per
-// device, with that device current.
+// the library (kNoAllocBytes of device memory per GPU, held from the first
+// plan created on that GPU to the end of the process) instead of the pool: no
+// allocation, so callers that must not allocate (code under graph capture
+// that may not grow the pool, tight serving loops) can use it.  The array's
+// name is a host-side stand-in; its device address comes from
+// hipGetSymbolAddress, once per device, with that device current, during plan
+// creation (the set-up every call needs anyway).
 constexpr size_t kNoAllocBytes = (size_t)64 << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 constexpr int kMaxDevices = 64;

@@ -62,7 +62,8 @@ int afft_gpu_execute(afft_gpu_plan* plan, const void* src_re, const void* src_im
  * scratch pool, serving loops and harnesses that forbid hipMalloc.  The plan's
  * scratch (and the copy of an input that overlaps the output) comes from a
  * fixed 64 MiB device buffer compiled into the library, one per GPU, held from
- * the library's load until the process ends; transforms run in groups that
+ * the first plan created on that GPU until the process ends (plan creation
+ * resolves it, so this call itself allocates nothing); transforms run in groups that
  * fit it, one launch per pass (the fused and pipelined schedules are not
  * used).  A plan whose single transform needs more than 64 MiB of scratch is
  * rejected with -1 and a message (afft_gpu_last_error), and nothing runs.
