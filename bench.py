@@ -247,14 +247,11 @@ def bench_queue(torch, dev, stream, reps=40):
     t0 = time.perf_counter()
     ntask = q.prepare(jobs)
     prep_ms = (time.perf_counter() - t0) * 1e3
-    # persistent work-groups per CU: the faster of 2 and 4 on this batch
-    per_cu = {}
-    for groups in (2, 4):
-        q.set_groups_per_cu(groups)
-        q.launch(stream)
-        per_cu[groups] = time_events(torch, stream, lambda: q.launch(stream), 10)
-    best_groups = min(per_cu, key=per_cu.get)
-    q.set_groups_per_cu(best_groups)
+    # the library's queue autotuner picks groups per CU and the chain lag on
+    # this batch and leaves the queue prepared with them
+    untuned_ms = time_events(torch, stream, lambda: q.launch(stream), 10)
+    tune_ms = q.tune(jobs, stream, 5)
+    groups, lag = q.config
     ms = time_events(torch, stream, lambda: q.launch(stream), 10)
     algb = sum(4 * (nin + nout) for (_, _, _, _, nin, nout) in lay)
     fl = sum((5.0 if k < 2 else 2.5) * (1 << lg) * lg for (k, lg, _, _, _, _) in lay)
@@ -281,7 +278,8 @@ def bench_queue(torch, dev, stream, reps=40):
     torch.cuda.empty_cache()
     return {"workload": "10000 x 1-D FP32 jobs, N=2^k k~U{6..16}, 70% c2c / 30% real, fwd/inv at random",
             "tasks": int(ntask), "prepare_host_ms": round(prep_ms, 3), "launch_ms": round(ms, 4),
-            "groups_per_cu": best_groups, "launch_ms_by_groups_per_cu": {k: round(v, 4) for k, v in per_cu.items()},
+            "tuned": {"groups_per_cu": groups, "lag": lag, "tune_ms": round(tune_ms, 4),
+                      "untuned_launch_ms": round(untuned_ms, 4)},
             "GFLOPs": round(fl / (ms * 1e-3) / 1e9, 1), "algorithmic_GBps": round(algb / (ms * 1e-3) / 1e9, 1),
             "graph_replay_p50_ms": round(float(np.percentile(lat, 50)), 4),
             "graph_replay_p99_ms": round(float(np.percentile(lat, 99)), 4), "graph_replays": reps}
@@ -362,6 +360,18 @@ def main():
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "step_frac": round(hbm_frac_step, 4),
                 "traffic_source": traffic_src}
+    # every pass of the step, not only the dominant one: algorithmic bytes of
+    # its launches (16 B per element read + written) over its in-step time
+    roofline["passes"] = []
+    for k in range(nf):
+        a, b = ranges(fwd)[k]
+        nl = 1 if b - a > 1 else fwd.launches(a)
+        t_launch = comb[k] / 2 / nl
+        bpl = 16.0 * n * (batch / nl)
+        roofline["passes"].append({"kernel": fwd.describe()[a], "launches_per_direction": nl,
+                                   "avg_launch_us": round(t_launch * 1e3, 2),
+                                   "achieved": round(bpl / (t_launch * 1e-3) / 1e9, 1),
+                                   "frac": round(bpl / (t_launch * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
     rp = rocprof_summary(desc_a)
     if rp:
         roofline["rocprof_avg_us"] = rp["avg_us"]

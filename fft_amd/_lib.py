@@ -36,6 +36,7 @@ AFFT_GPU_H_SYMBOLS = [
     "afft_gpu_queue_set_spin_limit", "afft_gpu_queue_set_groups_per_cu",
     "afft_gpu_plan_set_mode", "afft_gpu_mfma_selftest", "afft_gpu_plan_set_variant", "afft_gpu_plan_get_variant",
     "afft_gpu_plan_tune", "afft_gpu_plan_describe_variant", "afft_gpu_execute_noalloc",
+    "afft_gpu_queue_set_lag", "afft_gpu_queue_get_config", "afft_gpu_queue_tune",
 ]
 
 
@@ -129,6 +130,12 @@ def lib() -> ctypes.CDLL:
     L.afft_gpu_queue_set_spin_limit.restype = ctypes.c_int
     L.afft_gpu_queue_set_groups_per_cu.argtypes = [vp, ctypes.c_int]
     L.afft_gpu_queue_set_groups_per_cu.restype = ctypes.c_int
+    L.afft_gpu_queue_set_lag.argtypes = [vp, ctypes.c_int]
+    L.afft_gpu_queue_set_lag.restype = ctypes.c_int
+    L.afft_gpu_queue_get_config.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.afft_gpu_queue_get_config.restype = ctypes.c_int
+    L.afft_gpu_queue_tune.argtypes = [vp, vp, ctypes.c_size_t, vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.afft_gpu_queue_tune.restype = ctypes.c_int
     L.afft_gpu_plan_set_mode.argtypes = [vp, ctypes.c_int]
     L.afft_gpu_plan_set_mode.restype = ctypes.c_int
     L.afft_gpu_mfma_selftest.argtypes = []

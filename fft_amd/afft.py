@@ -367,6 +367,28 @@ class Queue:
         if _lib.lib().afft_gpu_queue_set_spin_limit(self.q, int(limit)) != 0:
             raise RuntimeError(_lib.last_error())
 
+    def set_lag(self, lag: int):
+        if _lib.lib().afft_gpu_queue_set_lag(self.q, int(lag)) != 0:
+            raise RuntimeError(_lib.last_error())
+
+    @property
+    def config(self):
+        """(groups per CU, lag) the next launch / prepare uses."""
+        g, lag = ctypes.c_int(), ctypes.c_int()
+        if _lib.lib().afft_gpu_queue_get_config(self.q, ctypes.byref(g), ctypes.byref(lag)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return g.value, lag.value
+
+    def tune(self, jobs, stream=None, iters: int = 3) -> float:
+        """Times the (groups per CU, lag) candidates on these jobs, keeps the
+        fastest and leaves the queue prepared; returns its ms per launch."""
+        arr, n = jobs
+        h = getattr(stream, "cuda_stream", stream)
+        ms = ctypes.c_float()
+        if _lib.lib().afft_gpu_queue_tune(self.q, arr, n, h, int(iters), ctypes.byref(ms)) != 0:
+            raise RuntimeError(_lib.last_error())
+        return ms.value
+
     def close(self):
         if self.q:
             _lib.lib().afft_gpu_queue_destroy(self.q)

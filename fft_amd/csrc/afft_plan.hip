@@ -39,6 +39,7 @@
 #include "afft_aux.h"
 #include "afft_fused.h"
 #include "afft_mfma.h"
+#include "afft_real4.h"
 #include "afft_registry.h"
 
 using namespace afgpu;
@@ -79,7 +80,11 @@ enum PassType {
   PT_REALROWFFT = 5,
   PT_REALIROWFFT = 6,
   PT_REALCOLPAIR = 7,
-  PT_REALICOLPAIR = 8
+  PT_REALICOLPAIR = 8,
+  PT_REAL4A = 9,
+  PT_REAL4B = 10,
+  PT_REAL4C = 11,
+  PT_REAL4D = 12
 };
 
 struct Side {
@@ -125,6 +130,14 @@ constexpr bool kFusedEnabled = false;
 // ones as a two-pass column four-step with balanced factors, whose tiles span
 // more contiguous columns than one long column tile can.
 constexpr idx_t kColDirectMax = 2048;
+// 1-D FP32 real transforms up to these lengths run as the real-input
+// four-step by default (variant real_pair = 2 asks for it at any length it
+// has kernels for); longer ones as the complex four-step with a real pass.
+// Measured over 2^16..2^23 with 256 MiB of reals per batch: r2c 0.31 vs 0.38
+// ms (2^16), 0.34 vs 0.43 (2^20), 0.43 vs 0.45 (2^22), 0.47 vs 0.49 (2^23);
+// c2r 0.29 vs 0.33 (2^16), 0.35 vs 0.40 (2^20), but 0.41 vs 0.35 (2^21).
+constexpr idx_t kReal4R2cMax = (idx_t)1 << 23;
+constexpr idx_t kReal4C2rMax = (idx_t)1 << 20;
 
 struct Variant {
   int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
@@ -510,6 +523,11 @@ struct Builder {
       r->swap_out = swo;
       return;
     }
+    // 1-D lengths beyond two passes (n > 4096 * 4096): three passes
+    if (inner == 1 && P.var.row_log2 <= 0 && n > (idx_t)max_col_len<T>() * 4096) {
+      fourstep3(in, out, outer_u, n, swi, swo, tmp_role);
+      return;
+    }
     // four-step along the axis: n = n1 * n2
     idx_t n2, n1;
     if (inner == 1) {
@@ -601,6 +619,83 @@ struct Builder {
       b->n_blo = (int)n1;
       b->b_per_user = outer_u * n1;
     }
+    b->swap_out = swo;
+  }
+
+  // 1-D c2c of n = p * q * n2 points in three passes: the four-step
+  // n = p * M (M = q * n2) whose second step, the M-point DFTs of the p rows,
+  // is itself a four-step.
+  //   pass 1: p-point columns (stride M) of the input, times W_n^(jM * kp),
+  //           into scratch rows kp of M;
+  //   pass 2: in place, q-point columns (stride n2) of each row kp, times
+  //           W_M^(j2 * kq);
+  //   pass 3: n2-point rows (kp, kq) with the transposed store of bin
+  //           kp + p * (kq + q * k2): for a block kq, the rows of consecutive
+  //           kp land next to each other (s_c = 1), the blocks p apart.
+  void fourstep3(Side in, Side out, idx_t outer_u, idx_t n, int swi, int swo, int tmp_role) {
+    const idx_t n2 = 2048;
+    const idx_t rem = n / n2;
+    const idx_t p = (idx_t)1 << ((ilog2(rem) + 1) / 2), q = rem / p, M = q * n2;
+    if (p > max_col_len<T>() || q > max_col_len<T>() || q < 2) {
+      set_err("transform length %lld is too long", (long long)n);
+      ok = false;
+      return;
+    }
+    const idx_t tens = outer_u * n;
+    need_scratch(tmp_role, tens);
+    auto bstride = [&](const Side& s, idx_t per_outer) -> idx_t {
+      if (s.role < R_SCR && outer_u == 1) return s.s_u;
+      return per_outer;
+    };
+    const int bits = (ilog2(n) + 1) / 2, bits_m = (ilog2(M) + 1) / 2;
+    const void* tw_n = get_table<T>(1, n, bits);
+    const void* tw_m = get_table<T>(1, M, bits_m);
+    if (!tw_n || !tw_m) {
+      ok = false;
+      return;
+    }
+    Side t;
+    scratch_side(t, tmp_role);
+    PassRec* a = add();
+    if (!a || !fft_pass(a, (int)p, 1, EPI_TW4, 0)) return;
+    a->in = in;
+    a->out = t;
+    a->in.s_j = a->out.s_j = M;
+    a->in.s_c = a->out.s_c = 1;
+    a->in.s_b = bstride(in, n);
+    a->out.s_b = n;
+    a->n_col = M;
+    a->b_per_user = outer_u;
+    a->swap_in = swi;
+    a->tw4_hi = tw_n;
+    a->tw4_lo = (const cx<T>*)tw_n + (n >> bits);
+    a->tw4_bits = bits;
+    PassRec* c = add();
+    if (!c || !fft_pass(c, (int)q, 1, EPI_TW4, 0)) return;
+    c->in = c->out = t;
+    c->in.s_j = c->out.s_j = n2;
+    c->in.s_c = c->out.s_c = 1;
+    c->in.s_b = c->out.s_b = M;
+    c->n_col = n2;
+    c->b_per_user = outer_u * p;
+    c->tw4_hi = tw_m;
+    c->tw4_lo = (const cx<T>*)tw_m + (M >> bits_m);
+    c->tw4_bits = bits_m;
+    PassRec* b = add();
+    if (!b || !fft_pass(b, (int)n2, 0, EPI_PLAIN, 1)) return;
+    b->in = t;
+    b->out = out;
+    b->in.s_j = 1;
+    b->in.s_c = M;
+    b->in.s_b = n;
+    b->in.s_b2 = n2;
+    b->out.s_j = p * q;
+    b->out.s_c = 1;
+    b->out.s_b = bstride(out, n);
+    b->out.s_b2 = p;
+    b->n_col = p;
+    b->n_blo = (int)q;
+    b->b_per_user = outer_u * q;
     b->swap_out = swo;
   }
 
@@ -707,6 +802,150 @@ struct Builder {
     r->rr_m = n1;
     r->real_tw = get_table<T>(2, h, 0);
     if (!r->real_tw) ok = false;
+  }
+
+  // 1-D FP32 r2c of N = 2h reals as a real-input four-step N = n1 * n2
+  // (afft_real4.hip): pass A over two real columns per complex column, rows
+  // k1 <= n1/2 of the twiddled column spectra into scratch; pass B the row
+  // DFTs, storing each bin once.  False when no kernels exist for the split.
+  bool r2c_real4(const Side& in, const Side& out, idx_t h) {
+    const idx_t N = 2 * h;
+    if (sizeof(T) != 4 || !(P.var.real_pair == 2 || (P.var.real_pair == 0 && N <= kReal4R2cMax))) return false;
+    const int lg = ilog2(N);
+    idx_t n2 = P.var.row_log2 > 0 && P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : (idx_t)1 << (lg - lg / 2);
+    if (n2 >= N) return false;
+    const idx_t n1 = N / n2;
+    int cta = 0, sva = 0, ctb = 0, svb = 0;
+    const int want_sv = P.var.sched < 0 ? 0 : (P.var.sched > 0 ? 1 : -1);
+    if (!real4_col_shape((int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), false, P.var.col_tile, want_sv, cta, sva) ||
+        !real4_row_shape((int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.rowt_tile, want_sv, ctb, svb) ||
+        (n2 / 2) % cta != 0)
+      return false;
+    const int bits = (lg + 1) / 2;
+    const void* tw4 = get_table<T>(1, N, bits);
+    const void* sa = get_table<T>(0, n1, sva);
+    const void* sb = get_table<T>(0, n2, svb);
+    if (!tw4 || !sa || !sb) {
+      ok = false;
+      return true;
+    }
+    const idx_t rows = n1 / 2 + 1;
+    need_scratch(R_SCR, rows * n2);
+    PassRec* a = add();
+    if (!a) return true;
+    a->type = PT_REAL4A;
+    a->L = (int)n1;
+    a->CT = cta;
+    a->sv = sva;
+    a->colmap = 1;
+    a->in = in;  // N reals as h interleaved complex pairs
+    a->in.s_j = n2 / 2;
+    a->in.s_c = 1;
+    a->in.s_b = in.s_u;
+    scratch_side(a->out, R_SCR);
+    a->out.s_j = n2;
+    a->out.s_c = 2;
+    a->out.s_b = rows * n2;
+    a->n_col = n2 / 2;
+    a->rr_h = h;
+    a->rr_m = n2;
+    a->stage_tw = sa;
+    a->hx = 1;
+    a->tw4_hi = tw4;
+    a->tw4_lo = (const cx<T>*)tw4 + (N >> bits);
+    a->tw4_bits = bits;
+    PassRec* b = add();
+    if (!b) return true;
+    b->type = PT_REAL4B;
+    b->L = (int)n2;
+    b->CT = ctb;
+    b->sv = svb;
+    b->hx = 1;
+    scratch_side(b->in, R_SCR);
+    b->in.s_j = 1;
+    b->in.s_c = n2;
+    b->in.s_b = rows * n2;
+    b->out = out;
+    b->out.s_j = n1;
+    b->out.s_c = 1;
+    b->out.s_b = out.s_u;
+    b->n_col = rows;
+    b->rr_h = h;
+    b->rr_m = n1;
+    b->stage_tw = sb;
+    return true;
+  }
+
+  // 1-D FP32 c2r of h+1 bins to N = 2h reals, the same split run backwards:
+  // pass B' the inverse row DFTs of rows k1 <= n1/2 (each row read from the
+  // stored bins and the conjugates of the mirrored ones) into scratch, pass A'
+  // the inverse column DFTs of two real columns per complex column.
+  bool c2r_real4(const Side& in, const Side& out, idx_t h) {
+    const idx_t N = 2 * h;
+    if (sizeof(T) != 4 || !(P.var.real_pair == 2 || (P.var.real_pair == 0 && N <= kReal4C2rMax))) return false;
+    const int lg = ilog2(N);
+    idx_t n2 = P.var.row_log2 > 0 && P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : (idx_t)1 << (lg - lg / 2);
+    if (n2 >= N) return false;
+    const idx_t n1 = N / n2;
+    int cta = 0, sva = 0, ctb = 0, svb = 0;
+    const int want_sv = P.var.sched < 0 ? 0 : (P.var.sched > 0 ? 1 : -1);
+    if (!real4_col_shape((int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), true, P.var.col_tile, want_sv, cta, sva) ||
+        !real4_row_shape((int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.rowt_tile, want_sv, ctb, svb) ||
+        (n2 / 2) % cta != 0)
+      return false;
+    const int bits = (lg + 1) / 2;
+    const void* tw4 = get_table<T>(1, N, bits);
+    const void* sa = get_table<T>(0, n1, sva);
+    const void* sb = get_table<T>(0, n2, svb);
+    if (!tw4 || !sa || !sb) {
+      ok = false;
+      return true;
+    }
+    const idx_t rows = n1 / 2 + 1;
+    need_scratch(R_SCR, rows * n2);
+    PassRec* b = add();
+    if (!b) return true;
+    b->type = PT_REAL4C;
+    b->L = (int)n2;
+    b->CT = ctb;
+    b->sv = svb;
+    b->hx = 1;
+    b->in = in;  // bins 0..h
+    b->in.s_j = n1;
+    b->in.s_c = 1;
+    b->in.s_b = in.s_u;
+    scratch_side(b->out, R_SCR);
+    b->out.s_j = 1;
+    b->out.s_c = n2;
+    b->out.s_b = rows * n2;
+    b->n_col = rows;
+    b->rr_h = h;
+    b->rr_m = n1;
+    b->stage_tw = sb;
+    b->tw4_hi = tw4;
+    b->tw4_lo = (const cx<T>*)tw4 + (N >> bits);
+    b->tw4_bits = bits;
+    PassRec* a = add();
+    if (!a) return true;
+    a->type = PT_REAL4D;
+    a->L = (int)n1;
+    a->CT = cta;
+    a->colmap = 1;
+    a->hx = 1;
+    a->sv = sva;
+    scratch_side(a->in, R_SCR);
+    a->in.s_j = n2;
+    a->in.s_c = 2;
+    a->in.s_b = rows * n2;
+    a->out = out;  // N reals as h interleaved complex pairs
+    a->out.s_j = n2 / 2;
+    a->out.s_c = 1;
+    a->out.s_b = out.s_u;
+    a->n_col = n2 / 2;
+    a->rr_h = h;
+    a->rr_m = n2;
+    a->stage_tw = sa;
+    return true;
   }
 
   // The inner c2c axes of an N-D r2c, on each of the h+1 rows of dst.
@@ -944,6 +1183,13 @@ struct Builder {
       for (int j = i + 1; j < P.npass; j++) reread = reread || P.pass[j].in.role == q.out.role;
       q.in_nt = P.var.stream_io && q.in.role == R_SRC;
       q.out_nt = P.var.stream_io && q.out.role == R_DST && !reread;
+      // the real-input four-step's row passes store (r2c) or load (c2r) the
+      // spectrum in runs of a tile's rows; a transform's N/2 + 1 bins rarely
+      // start on a line boundary, so neighbouring tiles share lines: cached
+      // accesses let those halves meet in L2 (measured 0.27 -> 0.16 ms for
+      // 64 x 2^20 with the XCD-contiguous tile order)
+      if (q.type == PT_REAL4B) q.out_nt = 0;
+      if (q.type == PT_REAL4C) q.in_nt = 0;
       q.xcd = P.var.xcd_order;
       q.mfma = 0;
       if (P.mode != afft_gpu_mode_fma && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
@@ -1056,13 +1302,15 @@ struct Builder {
           r->fold_user = 1;
           return ok;
         }
+        if (r2c_real4(in, out, h)) return ok;
         const idx_t n2 = row_len_1d(h);
-        if (n2 >= 2 && h / n2 >= 32) {
+        if (n2 >= 2 && h / n2 >= 32 && h <= (idx_t)max_col_len<T>() * 4096) {
           r2c_fourstep_tr(in, out, h);
           return ok;
         }
-        // short column pass (FP64 h = 16384): four-step with a transposed
-        // store, then the real pass in place
+        // short column pass (FP64 h = 16384), or h beyond two passes (the
+        // three-pass four-step): the c2c with a transposed store, then the
+        // real pass in place
         axis(in, out, 1, h, 1, 0, 0);
         Side rin = out, rout = out;
         rin.s_j = rout.s_j = 1;
@@ -1152,6 +1400,7 @@ struct Builder {
         r->swap_in = r->swap_out = 1;
         return ok;
       }
+      if (c2r_real4(in, out, h)) return ok;
       // long FP32 c2r whose four-step pass A has a kernel with the inverse
       // real pre-pass fused into its load: pass A reads the spectrum, pairs
       // bins k and h-k itself and no real pass runs on its own
@@ -1537,6 +1786,68 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     HIP_OK(real_colpair<T>(a, st));
     return true;
   }
+  if (r.type == PT_REAL4C || r.type == PT_REAL4D) {
+    Real4Args<T> a{};
+    const bool c = r.type == PT_REAL4C;
+    a.ct = r.CT;
+    a.sv = r.sv;
+    a.n1 = c ? r.rr_m : r.L;
+    a.n2 = c ? r.L : r.rr_m;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.scr_bs = (a.n1 / 2 + 1) * a.n2;
+    if (c) {
+      a.in = ire;
+      a.in_im = iim;
+      a.in_il = r.in.il;
+      a.in_bs = r.in.s_b;
+      a.in_nt = r.in_nt;
+      a.scr = reinterpret_cast<cx<T>*>(ore);
+      a.tw4_hi = (const cx<T>*)r.tw4_hi;
+      a.tw4_lo = (const cx<T>*)r.tw4_lo;
+      a.tw4_bits = r.tw4_bits;
+      HIP_OK(real4c<T>(a, st));
+    } else {
+      a.scr = const_cast<cx<T>*>(reinterpret_cast<const cx<T>*>(ire));
+      a.out_re = ore;
+      a.out_il = 1;
+      a.out_bs = 2 * r.out.s_b;  // complex pairs -> reals
+      a.out_nt = r.out_nt;
+      HIP_OK(real4d<T>(a, st));
+    }
+    return true;
+  }
+  if (r.type == PT_REAL4A || r.type == PT_REAL4B) {
+    Real4Args<T> a{};
+    const idx_t N = 2 * r.rr_h;
+    a.ct = r.CT;
+    a.sv = r.sv;
+    a.n1 = r.type == PT_REAL4A ? r.L : r.rr_m;
+    a.n2 = r.type == PT_REAL4A ? r.rr_m : r.L;
+    a.nbatch = cnt;
+    a.stage_tw = (const cx<T>*)r.stage_tw;
+    a.scr_bs = (a.n1 / 2 + 1) * a.n2;
+    if (r.type == PT_REAL4A) {
+      a.in = ire;
+      a.in_bs = 2 * r.in.s_b;  // complex pairs -> reals
+      a.scr = reinterpret_cast<cx<T>*>(ore);
+      a.in_nt = r.in_nt;
+      a.tw4_hi = (const cx<T>*)r.tw4_hi;
+      a.tw4_lo = (const cx<T>*)r.tw4_lo;
+      a.tw4_bits = r.tw4_bits;
+      (void)N;
+      HIP_OK(real4a<T>(a, st));
+    } else {
+      a.scr = const_cast<cx<T>*>(reinterpret_cast<const cx<T>*>(ire));
+      a.out_re = ore;
+      a.out_im = oim;
+      a.out_il = r.out.il;
+      a.out_bs = r.out.s_b;
+      a.out_nt = r.out_nt;
+      HIP_OK(real4b<T>(a, st));
+    }
+    return true;
+  }
   if (r.type == PT_REALTR) {
     RealTrArgs<T> a{};
     a.in = reinterpret_cast<const cx<T>*>(ire);
@@ -1819,14 +2130,23 @@ idx_t default_chunk(const Plan& P, size_t elem_bytes) {
   if (per == 0) return P.batch;
   if (P.var.chunk > 0) return P.var.chunk < P.batch ? P.var.chunk : P.batch;
   // Four-step groups: 64 MiB of scratch when pass A has long (>= 1024-point)
-  // columns; 256 MiB when they are short, where fewer, larger launches
-  // measured faster than keeping the scratch within the MALL
-  const bool short_cols = P.npass >= 1 && P.pass[0].type == PT_FFT && P.pass[0].colmap && P.pass[0].L <= 512;
+  // columns; 256 MiB when they are short, and for the real-input four-step,
+  // where fewer, larger launches measured faster than keeping the scratch
+  // within the MALL
+  const bool short_cols = P.npass >= 1 && ((P.pass[0].type == PT_FFT && P.pass[0].colmap && P.pass[0].L <= 512) ||
+                                           P.pass[0].type == PT_REAL4A || P.pass[0].type == PT_REAL4C);
   idx_t target = (idx_t)((short_cols ? 256u : 64u) << 20) / (idx_t)(per * elem_bytes);
   // pass A split into launches of the 64 MiB groups, pass B over kAutoSplit of them
-  if (!short_cols && auto_split(P) && target >= 1) target *= kAutoSplit;
+  const idx_t unit = !short_cols && auto_split(P) && target >= 1 ? target : 1;
+  if (unit > 1) target *= kAutoSplit;
   if (target < 1) target = 1;
-  return target < P.batch ? target : P.batch;
+  if (target >= P.batch) return P.batch;
+  // as many groups as the budget needs, of equal size (16 x r2c 2^22: 8 + 8
+  // transforms instead of 15 + 1, whose single-transform launches fill a
+  // fraction of the GPU), in whole pass-A launches where pass A is split
+  const idx_t groups = (P.batch + target - 1) / target;
+  const idx_t even = (P.batch + groups - 1) / groups;
+  return std::min(target, (even + unit - 1) / unit * unit);
 }
 
 // Validates arguments; returns squeezed dims in d / nd.  status: 0 ok,
@@ -2064,7 +2384,23 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
               cands.push_back(v);
             }
   // plans with a real pass that has a fused form also try the other form
-  for (int i = 0; i < P.npass; i++)
+  const bool long_1d_real =
+      sizeof(T) == 4 && P.kind >= 2 && P.ndim == 1 && P.dims[0] / 2 > (idx_t)max_row_len<T>();
+  if (long_1d_real) {
+    // 1-D FP32 real: the real-input four-step (with its wider column tiles)
+    // and the complex four-step with the real pass on its own or fused
+    for (int rp : {2, -1, 1}) {
+      Variant v;
+      v.real_pair = rp;
+      cands.push_back(v);
+      if (rp == 2)
+        for (int ct : {8, 16}) {
+          v.col_tile = ct;
+          cands.push_back(v);
+        }
+    }
+  }
+  for (int i = 0; i < P.npass && !long_1d_real; i++) {
     if (P.pass[i].type == PT_REALPAIR || P.pass[i].type == PT_REALTR || P.pass[i].type == PT_REALROWFFT ||
         P.pass[i].type == PT_REALIROWFFT || P.pass[i].type == PT_REALCOLPAIR ||
         P.pass[i].type == PT_REALICOLPAIR) {
@@ -2079,6 +2415,7 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
       }
       break;
     }
+  }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return false;
   float best = 1e30f;
@@ -2186,6 +2523,18 @@ int describe_pass(const Plan& P, int i, char* buf, size_t len) {
   else if (r.type == PT_REALICOLPAIR)
     snprintf(buf, len, "real_icolpair L=%d CT=%d h2=%d h=%lld m=%lld in(role=%d il=%d)", r.L, r.CT, r.n_blo,
              (long long)r.rr_h, (long long)r.rr_m, r.in.role, r.in.il);
+  else if (r.type == PT_REAL4A)
+    snprintf(buf, len, "real4a n1=%d CT=%d sv%d n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+             r.in.role, r.out.role);
+  else if (r.type == PT_REAL4B)
+    snprintf(buf, len, "real4b n2=%d CT=%d sv%d n1=%lld in(role=%d) out(role=%d il=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+             r.in.role, r.out.role, r.out.il);
+  else if (r.type == PT_REAL4C)
+    snprintf(buf, len, "real4c n2=%d CT=%d sv%d n1=%lld in(role=%d il=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+             r.in.role, r.in.il, r.out.role);
+  else if (r.type == PT_REAL4D)
+    snprintf(buf, len, "real4d n1=%d CT=%d sv%d n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+             r.in.role, r.out.role);
   else if (r.type == PT_REALCOLPAIR)
     snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
              (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);
@@ -2603,6 +2952,7 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
     o.tw4_bits = r.tw4_bits;
     o.tw4_shift = r.tw4_shift;
     o.tw4_blo = r.tw4_blo;
+    o.sv = r.sv;
     o.rr_h = r.rr_h;
     o.rr_m = r.rr_m;
     o.rr_inv = r.rr_inv;

@@ -63,7 +63,7 @@ constexpr int kQThreads = 256;
 constexpr int kQMaxLog2 = 16;
 constexpr int kBigN2 = 256;
 constexpr int kRealTile = 2048;  // bin pairs per real-pass tile
-constexpr int kLag = 8;          // jobs between consecutive steps of one chain (per XCD list)
+constexpr int kLag = 8;          // default jobs between consecutive steps of one chain (per XCD list)
 constexpr int kXcd = 8;          // XCDs of an MI355X: one chain list each
 constexpr int kCtr = 16;         // ints between counters (one 64-byte line each)
 
@@ -381,6 +381,7 @@ struct Queue {
   int ntasks = 0;
   int nchain = 0;                   // jobs run as tile chains
   int groups_per_cu = 4;            // persistent work-groups per CU (2 or 4; 4 measured faster)
+  int lag = kLag;                   // jobs between consecutive steps of one chain in a list
   int ncu = 256;
   long long spin_limit = 1LL << 26;  // dependency-wait iterations before the queue is flagged
   QTask* d_tasks = nullptr;
@@ -596,14 +597,16 @@ int afft_gpu_queue_prepare(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t 
       load[x] += (double)((size_t)1 << (jobs[big[c]].log2n));
     }
   }
-  // per XCD list: step s of its k-th job is listed at round k + s*kLag
+  // per XCD list: step s of its k-th job is listed at round k + s*lag (any
+  // lag >= 0 lists a step after the one it waits for)
+  const size_t lag = (size_t)q->lag;
   for (int x = 0; x < kXcd; x++) {
     q->hdr.list_off[x] = (int)q->h_tasks.size();
     const auto& js = on_xcd[x];
-    for (size_t round = 0; !js.empty() && round < js.size() + 3 * (size_t)kLag; round++)
+    for (size_t round = 0; !js.empty() && round < js.size() + 3 * lag; round++)
       for (int st = 0; st < 4; st++) {
-        if (round < (size_t)st * kLag) continue;
-        const size_t k = round - (size_t)st * kLag;
+        if (round < (size_t)st * lag) continue;
+        const size_t k = round - (size_t)st * lag;
         if (k >= js.size()) continue;
         const size_t c = js[k];
         if (st >= (int)steps[c].size()) continue;
@@ -745,6 +748,88 @@ int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* qh, int groups) {
   }
   q->groups_per_cu = groups;
   q->grid = q->ncu * groups;
+  return 0;
+}
+
+int afft_gpu_queue_set_lag(afft_gpu_queue* qh, int lag) {
+  Queue* q = get_queue(qh);
+  if (!q || lag < 0 || lag > 1024) {
+    queue_set_err("queue: not a queue or lag outside 0..1024");
+    return -1;
+  }
+  q->lag = lag;
+  return 0;
+}
+
+int afft_gpu_queue_get_config(const afft_gpu_queue* qh, int* groups_per_cu, int* lag) {
+  Queue* q = get_queue(qh);
+  if (!q) {
+    queue_set_err("queue: not a queue");
+    return -1;
+  }
+  if (groups_per_cu) *groups_per_cu = q->groups_per_cu;
+  if (lag) *lag = q->lag;
+  return 0;
+}
+
+// Times every (groups per CU, lag) candidate on this batch -- prepare, one
+// warm launch, `iters` timed launches -- and keeps the fastest whose launches
+// met every dependency; the queue is left prepared with it.
+int afft_gpu_queue_tune(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t njobs, void* stream, int iters,
+                        float* best_ms) {
+  Queue* q = get_queue(qh);
+  if (!q || iters < 1) {
+    queue_set_err("queue: not a queue or iters < 1");
+    return -1;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    queue_set_err("queue: event create failed");
+    return -1;
+  }
+  const int keep_groups = q->groups_per_cu, keep_lag = q->lag;
+  float best = 1e30f;
+  int bg = keep_groups, bl = keep_lag;
+  bool failed = false;
+  for (int groups : {4, 2}) {
+    for (int lag : {kLag, 2, 32}) {
+      if (afft_gpu_queue_set_groups_per_cu(qh, groups) != 0 || afft_gpu_queue_set_lag(qh, lag) != 0 ||
+          afft_gpu_queue_prepare(qh, jobs, njobs) < 0 || afft_gpu_queue_launch(qh, stream) != 0) {
+        failed = true;
+        break;
+      }
+      float ms = 0.f;
+      if (hipEventRecord(e0, st) != hipSuccess) failed = true;
+      for (int i = 0; i < iters && !failed; i++)
+        if (afft_gpu_queue_launch(qh, stream) != 0) failed = true;
+      if (failed || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        failed = true;
+        break;
+      }
+      if (afft_gpu_queue_status(qh) != 0) continue;  // a dependency wait ran out: not a candidate
+      ms /= (float)iters;
+      if (ms < best) {
+        best = ms;
+        bg = groups;
+        bl = lag;
+      }
+    }
+    if (failed) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  q->groups_per_cu = bg;
+  q->grid = q->ncu * bg;
+  q->lag = bl;
+  if (failed) {
+    afft_gpu_queue_set_groups_per_cu(qh, keep_groups);
+    q->lag = keep_lag;
+    return -1;
+  }
+  if (afft_gpu_queue_prepare(qh, jobs, njobs) < 0) return -1;
+  if (best_ms) *best_ms = best;
   return 0;
 }
 

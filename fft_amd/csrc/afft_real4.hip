@@ -1,0 +1,552 @@
+// afft_real4.hip -- 1-D r2c of a long real sequence as a real-input four-step
+// (N = n1 * n2 reals, two passes over half-size data; see afft_real4.h).
+//
+// afft computes its r2c as an N/2-point complex FFT of the packed sequence
+// x[2j] + i x[2j+1] followed by a real post-pass pairing bins k and N/2 - k
+// (onedim.hpp:1031-1108, 1175-1189).  On the GPU that is three round trips
+// through memory for long N (the two four-step passes and the post-pass, which
+// needs the bins of two different rows).  Here the real input enters the
+// four-step directly:
+//   pass A  reads two adjacent real columns 2c, 2c+1 of the n1 x n2 matrix as
+//           one complex column (they are adjacent floats), runs its n1-point
+//           DFT Z, separates the two real columns' spectra through LDS,
+//             A[k1] = (Z[k1] + conj Z[n1 - k1]) / 2,
+//             B[k1] = (Z[k1] - conj Z[n1 - k1]) / (2i),
+//           multiplies by W_N^(2c*k1) and W_N^((2c+1)*k1) and stores rows
+//           k1 = 0..n1/2 only (the others are their conjugates);
+//   pass B  runs the n2-point DFTs of those rows and stores, through LDS,
+//           X[k1 + n1*k2] for the bins up to N/2 and, for 0 < k1 < n1/2, the
+//           bins of row n1 - k1 as conj X[k1 + n1*(n2 - 1 - k2)].
+// Memory traffic: N reals in, N/2 complex through scratch, N/2 + 1 complex
+// out -- two round trips instead of three.
+#include "afft_real4.h"
+
+namespace afgpu {
+
+// Work-group ids in XCD-contiguous order: the hardware deals consecutive
+// work-groups to the 8 XCDs in turn; this mapping gives XCD x a contiguous
+// range of tiles, so neighbouring row tiles -- which share the cache lines at
+// their edges when a transform's output does not start on a line boundary --
+// run on one XCD and meet in its L2.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned id, unsigned n) {
+  const unsigned per = n / 8, rem = n % 8, x = id % 8;
+  return x * per + (x < rem ? x : rem) + id / 8;
+}
+
+template <typename T, int L, int CT, int SV, bool HX>
+struct R4AShape {
+  using PS = PassShape<L, CT, SV>;
+  static constexpr int E = PS::E;
+  static constexpr int TPT = PS::TPT;
+  static constexpr int NT = PS::NT;
+  static constexpr size_t lds = PS::template lds_bytes<T, HX>();
+  static constexpr int groups = 2 * lds <= 160 * 1024 ? 2 : 1;
+  static constexpr int min_waves = groups * NT / 256 > 1 ? groups * NT / 256 : 1;
+};
+
+template <typename T, int L, int CT, int SV, bool HX>
+struct R4BShape {
+  using PS = PassShape<L, CT, SV, HX ? (int)sizeof(T) : 2 * (int)sizeof(T)>;
+  static constexpr int E = PS::E;
+  static constexpr int TPT = PS::TPT;
+  static constexpr int NT = PS::NT;
+  static constexpr int LL = PS::LL;
+  static constexpr size_t lds = PS::template lds_bytes<T, HX>();
+  static constexpr int groups = 2 * lds <= 160 * 1024 ? 2 : 1;
+  static constexpr int min_waves = groups * NT / 256 > 1 ? groups * NT / 256 : 1;
+};
+
+// Pass A: CT packed columns (2 CT real columns) of L = n1 points per group.
+template <typename T, int L, int CT, int SV, bool HX>
+__global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T, L, CT, SV, HX>::min_waves))
+    real4a_kernel(Real4Args<T> a) {
+  using S = R4AShape<T, L, CT, SV, HX>;
+  constexpr int E = S::E;
+  constexpr int TPT = S::TPT;
+  constexpr int MH = E / 2;  // registers m <= MH hold bins k1 <= L/2 (m == MH: thread 0 only)
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const int tid = threadIdx.x;
+  const int c = tid % CT, t = tid / CT;
+  const idx_t npc = a.n2 / 2;  // packed columns per transform
+  const int col_tiles = (int)(npc / CT);
+  const idx_t b = blockIdx.x / col_tiles;
+  const idx_t col = (idx_t)(blockIdx.x - b * col_tiles) * CT + c;
+  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
+  auto latf = [&](int i) -> T& { return ldsf[lds_pad(i) * CT + c]; };
+
+  // element j1 of packed column col: the reals x[j1*n2 + 2col], x[j1*n2 + 2col + 1]
+  const T* src = a.in + b * a.in_bs + 2 * col;
+  cx<T> v[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    v[m] = ld_elem<T>(src, nullptr, 1, (idx_t)(t + m * TPT) * npc, a.in_nt);
+  });
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true, CT>(v, t, latf, a.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false, CT>(v, t, lat, a.stage_tw);
+
+  // partner Z[n1 - k1] of every bin k1 = t + m*TPT <= L/2
+  cx<T> zp[MH + 1];
+  auto partner = [&](int m) AFGPU_INLINE { return (L - (t + m * TPT)) & (L - 1); };
+  if constexpr (!HX) {
+    static_for<E>([&](auto mc) AFGPU_INLINE { lat(t + decltype(mc)::value * TPT) = v[decltype(mc)::value]; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      zp[m] = lat(partner(m));
+    });
+  } else {
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      zp[m].x = latf(partner(m));
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      zp[m].y = latf(partner(m));
+    });
+  }
+
+  // unpack, twiddle, store rows k1 <= n1/2 of Y (columns 2col, 2col + 1)
+  const unsigned mask = (1u << a.tw4_bits) - 1u;
+  cx<T>* dst = a.scr + b * a.scr_bs + 2 * col;
+  const T h = T(0.5);
+  static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    const int k1 = t + m * TPT;
+    if (k1 <= L / 2) {
+      const cx<T> zc = conjg(zp[m]);
+      const cx<T> s = v[m] + zc, d = v[m] - zc;
+      cx<T> A = {s.x * h, s.y * h};
+      cx<T> B = mul_neg_i(cx<T>{d.x * h, d.y * h});
+      const unsigned e0 = (unsigned)(2 * col) * (unsigned)k1;
+      A = cmul(A, tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e0));
+      B = cmul(B, tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e0 + (unsigned)k1));
+      cx<T>* row = dst + (idx_t)k1 * a.n2;
+      row[0] = A;
+      row[1] = B;
+    }
+  });
+}
+
+template <typename T>
+__device__ __forceinline__ void r4_store(const Real4Args<T>& a, idx_t e, cx<T> z) {
+  if (a.out_il)
+    st_elem<T>(a.out_re, nullptr, 1, e, z, a.out_nt);
+  else
+    st_elem<T>(a.out_re, a.out_im, 0, e, z, a.out_nt);
+}
+
+// Pass B: CT rows of L = n2 points per group (rows 0..n1/2), transposed store
+// of the own and the mirrored bins.
+template <typename T, int L, int CT, int SV, bool HX>
+__global__ void __launch_bounds__((R4BShape<T, L, CT, SV, HX>::NT), (R4BShape<T, L, CT, SV, HX>::min_waves))
+    real4b_kernel(Real4Args<T> a) {
+  using S = R4BShape<T, L, CT, SV, HX>;
+  constexpr int E = S::E;
+  constexpr int TPT = S::TPT;
+  constexpr int LL = S::LL;
+  constexpr int MH = E / 2;  // registers m <= MH of the store phase hold bins k2 <= L/2
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const int tid = threadIdx.x;
+  const int c = tid / TPT, t = tid % TPT;
+  const idx_t half = a.n1 / 2;
+  const idx_t nrows = half + 1;
+  const int row_tiles = (int)((nrows + CT - 1) / CT);
+  const unsigned bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const idx_t b = bid / row_tiles;
+  const idx_t row0 = (idx_t)(bid - b * row_tiles) * CT;
+  const idx_t row = row0 + c;
+  auto lat = [&](int i) -> cx<T>& { return lds[c * LL + lds_pad(i)]; };
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
+
+  cx<T> v[E];
+  const cx<T>* src = a.scr + b * a.scr_bs + row * a.n2;
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    v[m] = row < nrows ? src[t + m * TPT] : cx<T>{T(0), T(0)};
+  });
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false, 1>(v, t, lat, a.stage_tw);
+
+  // store phase: lane c2 = tid % CT takes row row0 + c2, bins k2 = t2 + m*TPT
+  const int c2 = tid % CT, t2 = tid / CT;
+  const idx_t r2 = row0 + c2;
+  const idx_t n1 = a.n1;
+  const idx_t ob = b * a.out_bs;
+  auto own = [&](int k2) AFGPU_INLINE { return r2 <= half && (k2 < L / 2 || (r2 == 0 && k2 == L / 2)); };
+  auto mirror = [&](int k2) AFGPU_INLINE { return r2 > 0 && r2 < half && k2 < L / 2; };
+  if constexpr (!HX) {
+    static_for<E>([&](auto mc) AFGPU_INLINE { lat(t + decltype(mc)::value * TPT) = v[decltype(mc)::value]; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int k2 = t2 + m * TPT;
+      if (own(k2)) r4_store(a, ob + r2 + n1 * k2, lds[c2 * LL + lds_pad(k2)]);
+      if (mirror(k2)) r4_store(a, ob + (n1 - r2) + n1 * k2, conjg(lds[c2 * LL + lds_pad(L - 1 - k2)]));
+    });
+  } else {
+    T xo[MH + 1], xm[MH + 1];
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int k2 = t2 + m * TPT;
+      xo[m] = ldsf[c2 * LL + lds_pad(k2)];
+      xm[m] = ldsf[c2 * LL + lds_pad(L - 1 - k2)];
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
+    __syncthreads();
+    static_for<MH + 1>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int k2 = t2 + m * TPT;
+      if (own(k2)) r4_store(a, ob + r2 + n1 * k2, cx<T>{xo[m], ldsf[c2 * LL + lds_pad(k2)]});
+      if (mirror(k2)) r4_store(a, ob + (n1 - r2) + n1 * k2, cx<T>{xm[m], -ldsf[c2 * LL + lds_pad(L - 1 - k2)]});
+    });
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> r4_load_in(const Real4Args<T>& a, idx_t e) {
+  return a.in_il ? ld_elem<T>(a.in, nullptr, 1, e, a.in_nt) : ld_elem<T>(a.in, a.in_im, 0, e, a.in_nt);
+}
+
+// c2r pass B': CT rows of L = n2 points per group (rows 0..n1/2), loaded
+// through LDS from the own and the mirrored bins, inverse DFT, W_N^-(j2*k1),
+// rows of D stored to scratch.
+template <typename T, int L, int CT, int SV, bool HX>
+__global__ void __launch_bounds__((R4BShape<T, L, CT, SV, HX>::NT), (R4BShape<T, L, CT, SV, HX>::min_waves))
+    real4c_kernel(Real4Args<T> a) {
+  using S = R4BShape<T, L, CT, SV, HX>;
+  constexpr int E = S::E;
+  constexpr int TPT = S::TPT;
+  constexpr int LL = S::LL;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const int tid = threadIdx.x;
+  const int c = tid / TPT, t = tid % TPT;
+  const idx_t half = a.n1 / 2;
+  const idx_t nrows = half + 1;
+  const int row_tiles = (int)((nrows + CT - 1) / CT);
+  const unsigned bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const idx_t b = bid / row_tiles;
+  const idx_t row0 = (idx_t)(bid - b * row_tiles) * CT;
+  const idx_t row = row0 + c;
+  const idx_t n1 = a.n1;
+  const idx_t ib = b * a.in_bs;
+  auto lat = [&](int i) -> cx<T>& { return lds[c * LL + lds_pad(i)]; };
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
+
+  // load phase: lane c2 = tid % CT takes row row0 + c2, bins k2 = t2 + m*TPT
+  // (consecutive lanes read consecutive bins of one k2)
+  const int c2 = tid % CT, t2 = tid / CT;
+  const idx_t r2 = row0 + c2;
+  cx<T> w[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    const int k2 = t2 + m * TPT;
+    cx<T> z = {T(0), T(0)};
+    if (r2 <= half) {
+      if (k2 < L / 2 || (r2 == 0 && k2 == L / 2)) {
+        z = r4_load_in(a, ib + r2 + n1 * k2);
+        if (r2 == 0 && (k2 == 0 || k2 == L / 2)) z.y = T(0);  // bins 0 and N/2: real parts only
+      } else if (r2 == 0) {
+        z = conjg(r4_load_in(a, ib + n1 * (L - k2)));
+      } else {
+        z = conjg(r4_load_in(a, ib + (n1 - r2) + n1 * (L - 1 - k2)));
+      }
+    }
+    w[m] = z;
+  });
+  cx<T> v[E];
+  if constexpr (!HX) {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      lds[c2 * LL + lds_pad(t2 + m * TPT)] = w[m];
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = swapri(lat(t + decltype(mc)::value * TPT)); });
+    __syncthreads();
+  } else {
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      ldsf[c2 * LL + lds_pad(t2 + m * TPT)] = w[m].x;
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value].y = latf(t + decltype(mc)::value * TPT); });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      ldsf[c2 * LL + lds_pad(t2 + m * TPT)] = w[m].y;
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value].x = latf(t + decltype(mc)::value * TPT); });
+    __syncthreads();
+  }
+  // inverse DFT as swap(DFT(swap(.))): v holds swap(R[row][k2])
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false, 1>(v, t, lat, a.stage_tw);
+  if (row <= half) {
+    const unsigned mask = (1u << a.tw4_bits) - 1u;
+    cx<T>* dst = a.scr + b * a.scr_bs + row * a.n2;
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int j2 = t + m * TPT;
+      const cx<T> wt = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)j2 * (unsigned)row);
+      dst[j2] = cmulc(swapri(v[m]), wt);
+    });
+  }
+}
+
+// c2r pass A': CT packed columns of L = n1 points per group: the column
+// spectra of two real columns at once, inverse DFT, two columns of reals.
+template <typename T, int L, int CT, int SV, bool HX>
+__global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T, L, CT, SV, HX>::min_waves))
+    real4d_kernel(Real4Args<T> a) {
+  using S = R4AShape<T, L, CT, SV, HX>;
+  constexpr int E = S::E;
+  constexpr int TPT = S::TPT;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const int tid = threadIdx.x;
+  const int c = tid % CT, t = tid / CT;
+  const idx_t npc = a.n2 / 2;
+  const int col_tiles = (int)(npc / CT);
+  const idx_t b = blockIdx.x / col_tiles;
+  const idx_t col = (idx_t)(blockIdx.x - b * col_tiles) * CT + c;
+  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
+  auto latf = [&](int i) -> T& { return ldsf[lds_pad(i) * CT + c]; };
+
+  // Z[k1] = D[k1][2col] + i D[k1][2col + 1]; rows k1 > n1/2 from conj D[n1 - k1]
+  const cx<T>* src = a.scr + b * a.scr_bs + 2 * col;
+  cx<T> v[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    const int k1 = t + m * TPT;
+    const bool lo = k1 <= L / 2;
+    const cx<T>* row = src + (idx_t)(lo ? k1 : L - k1) * a.n2;
+    cx<T> da = row[0], db = row[1];
+    if (!lo) {
+      da = conjg(da);
+      db = conjg(db);
+    }
+    // swap(da + i db) for the inverse DFT
+    v[m] = cx<T>{da.y + db.x, da.x - db.y};
+  });
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true, CT>(v, t, latf, a.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false, CT>(v, t, lat, a.stage_tw);
+  // z[j1] = swap(v) = x[j1*n2 + 2col] + i x[j1*n2 + 2col + 1]
+  T* dst = a.out_re + b * a.out_bs + 2 * col;
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    st_elem<T>(dst, nullptr, 1, (idx_t)(t + m * TPT) * npc, swapri(v[m]), a.out_nt);
+  });
+}
+
+// ---- launchers --------------------------------------------------------------
+template <typename T, int L, int CT, int SV, bool HX>
+static hipError_t launch_r4a(const Real4Args<T>& a, hipStream_t st) {
+  using S = R4AShape<T, L, CT, SV, HX>;
+  auto k = real4a_kernel<T, L, CT, SV, HX>;
+  if (S::lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  const idx_t npc = a.n2 / 2;
+  if (npc % CT != 0 || a.n1 != L) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (npc / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(S::NT), S::lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename T, int L, int CT, int SV, bool HX>
+static hipError_t launch_r4b(const Real4Args<T>& a, hipStream_t st) {
+  using S = R4BShape<T, L, CT, SV, HX>;
+  auto k = real4b_kernel<T, L, CT, SV, HX>;
+  if (S::lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  if (a.n2 != L) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * ((a.n1 / 2 + 1 + CT - 1) / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(S::NT), S::lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename T, int L, int CT, int SV, bool HX>
+static hipError_t launch_r4c(const Real4Args<T>& a, hipStream_t st) {
+  using S = R4BShape<T, L, CT, SV, HX>;
+  auto k = real4c_kernel<T, L, CT, SV, HX>;
+  if (S::lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  if (a.n2 != L) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * ((a.n1 / 2 + 1 + CT - 1) / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(S::NT), S::lds, st, a);
+  return hipGetLastError();
+}
+
+template <typename T, int L, int CT, int SV, bool HX>
+static hipError_t launch_r4d(const Real4Args<T>& a, hipStream_t st) {
+  using S = R4AShape<T, L, CT, SV, HX>;
+  auto k = real4d_kernel<T, L, CT, SV, HX>;
+  if (S::lds > 65536) {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::lds);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+  }
+  const idx_t npc = a.n2 / 2;
+  if (npc % CT != 0 || a.n1 != L) return hipErrorInvalidValue;
+  const idx_t blocks = a.nbatch * (npc / CT);
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(S::NT), S::lds, st, a);
+  return hipGetLastError();
+}
+
+// Compiled shapes (length, tile, stage schedule), the default of each length
+// first.  Column passes (A, A'): packed columns per group -- 16 read 128-byte
+// row segments of the reals and write 256-byte ones.  Row passes (B, B'): rows
+// per group.  All use the half exchange, so two groups of <= 80 KiB share a CU.
+struct R4Shape {
+  int L, CT, SV;
+};
+// 2048-point columns: 16 per group for the r2c pass A (one group per CU,
+// 128-byte segments: 16 x r2c 2^22 0.469 -> 0.427 ms), 8 for the c2r pass A'
+// (two groups per CU: 0.122 against 0.155 ms for 16 x 2^22)
+constexpr R4Shape kR4ColA[] = {{128, 16, 0}, {256, 16, 0}, {512, 16, 1}, {1024, 16, 1}, {2048, 16, 1}, {2048, 8, 1}};
+constexpr R4Shape kR4ColD[] = {{128, 16, 0}, {256, 16, 0}, {512, 16, 1}, {1024, 16, 1}, {2048, 8, 1}, {2048, 16, 1}};
+constexpr R4Shape kR4Row[] = {{256, 16, 0}, {512, 16, 0}, {1024, 8, 0},  {1024, 16, 1}, {2048, 8, 0},
+                              {2048, 8, 1}, {2048, 16, 1}, {4096, 4, 0}};
+
+template <size_t N>
+static bool r4_pick(const R4Shape (&tab)[N], int L, int want_ct, int want_sv, int& ct, int& sv) {
+  const R4Shape* first = nullptr;
+  for (const R4Shape& s : tab) {
+    if (s.L != L) continue;
+    if (!first) first = &s;
+    if ((want_ct <= 0 || s.CT == want_ct) && (want_sv < 0 || s.SV == want_sv)) {
+      ct = s.CT;
+      sv = s.SV;
+      return true;
+    }
+  }
+  if (!first) return false;
+  ct = first->CT;
+  sv = first->SV;
+  return true;
+}
+
+bool real4_col_shape(int elem_bytes, int n1, bool inverse, int want_ct, int want_sv, int& ct, int& sv) {
+  return elem_bytes == 4 && (inverse ? r4_pick(kR4ColD, n1, want_ct, want_sv, ct, sv)
+                                     : r4_pick(kR4ColA, n1, want_ct, want_sv, ct, sv));
+}
+bool real4_row_shape(int elem_bytes, int n2, int want_ct, int want_sv, int& ct, int& sv) {
+  return elem_bytes == 4 && r4_pick(kR4Row, n2, want_ct, want_sv, ct, sv);
+}
+
+#define AFGPU_R4_COL(F, LL, CTT, SVV) \
+  if (a.n1 == LL && a.ct == CTT && a.sv == SVV) return F<float, LL, CTT, SVV, true>(a, st);
+#define AFGPU_R4_ROW(F, LL, CTT, SVV) \
+  if (a.n2 == LL && a.ct == CTT && a.sv == SVV) return F<float, LL, CTT, SVV, true>(a, st);
+#define AFGPU_R4_COLS(F)            \
+  AFGPU_R4_COL(F, 128, 16, 0)       \
+  AFGPU_R4_COL(F, 256, 16, 0)       \
+  AFGPU_R4_COL(F, 512, 16, 1)       \
+  AFGPU_R4_COL(F, 1024, 16, 1)      \
+  AFGPU_R4_COL(F, 2048, 8, 1)       \
+  AFGPU_R4_COL(F, 2048, 16, 1)
+#define AFGPU_R4_ROWS(F)            \
+  AFGPU_R4_ROW(F, 256, 16, 0)       \
+  AFGPU_R4_ROW(F, 512, 16, 0)       \
+  AFGPU_R4_ROW(F, 1024, 8, 0)       \
+  AFGPU_R4_ROW(F, 1024, 16, 1)      \
+  AFGPU_R4_ROW(F, 2048, 8, 0)       \
+  AFGPU_R4_ROW(F, 2048, 8, 1)       \
+  AFGPU_R4_ROW(F, 2048, 16, 1)      \
+  AFGPU_R4_ROW(F, 4096, 4, 0)
+
+template <>
+hipError_t real4a<float>(const Real4Args<float>& a, hipStream_t st) {
+  AFGPU_R4_COLS(launch_r4a)
+  return hipErrorInvalidValue;
+}
+template <>
+hipError_t real4b<float>(const Real4Args<float>& a, hipStream_t st) {
+  AFGPU_R4_ROWS(launch_r4b)
+  return hipErrorInvalidValue;
+}
+template <>
+hipError_t real4c<float>(const Real4Args<float>& a, hipStream_t st) {
+  AFGPU_R4_ROWS(launch_r4c)
+  return hipErrorInvalidValue;
+}
+template <>
+hipError_t real4d<float>(const Real4Args<float>& a, hipStream_t st) {
+  AFGPU_R4_COLS(launch_r4d)
+  return hipErrorInvalidValue;
+}
+#undef AFGPU_R4_COLS
+#undef AFGPU_R4_ROWS
+#undef AFGPU_R4_COL
+#undef AFGPU_R4_ROW
+
+template <>
+hipError_t real4c<double>(const Real4Args<double>&, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+template <>
+hipError_t real4d<double>(const Real4Args<double>&, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t real4a<double>(const Real4Args<double>&, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+template <>
+hipError_t real4b<double>(const Real4Args<double>&, hipStream_t) {
+  return hipErrorInvalidValue;
+}
+
+}  // namespace afgpu

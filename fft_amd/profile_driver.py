@@ -7,7 +7,9 @@ under profiles/ (the exact command sits in each summary).
     python -m fft_amd.profile_driver queue [iters]
 
 Workloads: c2c_2p20 (64 x 2^20), c2c_2p16 (1024 x 2^16), c2c_2p12, r2c_2p22
-(16 x 2^22), r2c_2p20 (64 x 2^20; r2c_2p20_tr with the unfused real pass), c2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
+(16 x 2^22), r2c_2p20 (64 x 2^20; r2c_2p20_tr with the complex four-step and
+its real pass), c2r_2p20 (64 x 2^20), c2r_2p22 (16 x 2^22), c2c_2p25 (2 x 2^25,
+three passes), c2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
 and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), copy (512
 MiB device copy, the counter calibration), queue (10,000 uneven jobs).
 """
@@ -68,6 +70,9 @@ def main():
             "c2c_256c": ("c", (256, 256, 256), 1),
             "r2c_8192s": ("r", (8192, 8192), 1),
             "c2r_8192s": ("ri", (8192, 8192), 1),
+            "c2r_2p20": ("ri", (1 << 20,), 64),
+            "c2r_2p22": ("ri", (1 << 22,), 16),
+            "c2c_2p25": ("c", (1 << 25,), 2),
             "c2c_2p16_4096": ("c", (1 << 16,), 4096),
             "c2c_2p16_4096_mfma": ("c", (1 << 16,), 4096),
         }

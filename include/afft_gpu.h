@@ -134,15 +134,19 @@ typedef struct
                         (or, where that has no kernel, rows of up to 4096
                         points) and a 1024..8192-point last axis of a 2-D
                         transform, inside the row FFTs; with -1 as a pass of
-                        its own.  FP32 1-D c2r with
-                        2^20 <= N <= 2^22: 1 runs the inverse real pre-pass in the
-                        load of four-step pass A (measured slower); 0 or -1: a
-                        pass of its own.  FP32 1-D r2c,
-                        2^16 <= N <= 2^24: 1 runs pass B of the four-step with the
-                        real post-pass and the output transposition fused in (rows
-                        k1 and n1-k1 per work-group; half the scratch, measured
-                        5-10 % slower); 0 (default) or -1: a row pass into a second
-                        scratch, then the transposing real pass */
+                        its own.  FP32 1-D real transforms with N > 2^15:
+                        2 runs the real-input four-step (two passes over N/2
+                        complex: pass A reads two real columns as one complex
+                        column, pass B stores each bin once; 2^16 <= N <= 2^23),
+                        the default for r2c up to 2^23 and c2r up to 2^20.
+                        Longer ones (or -1) run the complex four-step of the
+                        N/2 packed points with the real pass on its own: r2c
+                        as a row pass into a second scratch and the transposing
+                        real pass, c2r with the inverse real pre-pass first.
+                        1: r2c with pass B, the real post-pass and the
+                        transposition fused (rows k1 and n1-k1 per work-group),
+                        c2r (2^20 <= N <= 2^22) with the pre-pass in the load of
+                        pass A -- both measured slower */
   int32_t a_split;   /* two-pass plans (pass A: input -> scratch, pass B: scratch ->
                         output): pass A of each group runs in this many launches,
                         pass B in one (0 or 1: one) */
@@ -239,6 +243,20 @@ int afft_gpu_queue_set_spin_limit(afft_gpu_queue* q, long long limit);
  * lane, some spills) or 2 (registers for the largest tiles, half the tiles in
  * flight). */
 int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* q, int groups);
+/* Jobs between consecutive steps of one long job in a task list (default 8;
+ * 0..1024): larger lags let more of a chain's tiles finish before a dependent
+ * tile is popped, smaller ones keep a job's data in cache.  Takes effect at
+ * the next prepare. */
+int afft_gpu_queue_set_lag(afft_gpu_queue* q, int lag);
+int afft_gpu_queue_get_config(const afft_gpu_queue* q, int* groups_per_cu, int* lag);
+/* Autotuning for a batch (no counterpart in afft, which has no queue): times
+ * each (groups per CU, lag) candidate on these jobs -- prepare, one warm
+ * launch, `iters` timed launches on `stream` -- keeps the fastest whose
+ * launches met every dependency and leaves the queue prepared with it.
+ * best_ms (optional) receives its time per launch.  The jobs' outputs are
+ * written by every candidate (the same values).  Returns 0, or -1. */
+int afft_gpu_queue_tune(afft_gpu_queue* q, const afft_gpu_job* jobs, size_t njobs, void* stream, int iters,
+                        float* best_ms);
 void afft_gpu_queue_destroy(afft_gpu_queue* q);
 
 /* ---- plan inspection (host only, no GPU needed) ---------------------------- */
@@ -283,6 +301,8 @@ typedef struct
   int32_t rr_inv;
   int32_t tw4_blo; /* 1: the four-step twiddle's column index n2 is the low batch index
                       (b % n_blo, N = L * n_blo) instead of c >> tw4_shift */
+  int32_t sv;      /* stage schedule of the pass kernel (0: default radices, 1: radix-32) */
+  int32_t pad_;
 } afft_gpu_pass_info;
 
 typedef struct

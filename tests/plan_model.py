@@ -33,7 +33,8 @@ class PassInfo(ctypes.Structure):
                 ("fold_user", ctypes.c_int32), ("n_blo", ctypes.c_int32), ("b_per_user", ctypes.c_int64),
                 ("swap_in", ctypes.c_int32), ("swap_out", ctypes.c_int32), ("tw4_bits", ctypes.c_int32),
                 ("tw4_shift", ctypes.c_int32), ("rr_h", ctypes.c_int64), ("rr_m", ctypes.c_int64),
-                ("rr_inv", ctypes.c_int32), ("tw4_blo", ctypes.c_int32)]
+                ("rr_inv", ctypes.c_int32), ("tw4_blo", ctypes.c_int32), ("sv", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -454,6 +455,126 @@ def _real_irowfft(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, ob + rows.reshape(1, -1, 1) * p.out.s_c + j * p.out.s_j, y, f"pass {pi} store")
 
 
+def _real4a(p, bufs, start, cnt, pi):
+    """Pass A of the real-input four-step (afft_real4.hip real4a_kernel):
+    packed column c (reals 2c, 2c+1 of each row j1 of the n1 x n2 matrix)
+    -> n1-point DFT Z -> A = (Z + conj Z[-k1]) / 2, B = (Z - conj Z[-k1]) / 2i
+    -> times W_N^(2c*k1), W_N^((2c+1)*k1) -> rows k1 = 0..n1/2 of the scratch,
+    columns 2c and 2c+1 (row k1 at k1*out.s_j, column at c*out.s_c)."""
+    n1, n2, N = p.L, p.rr_m, 2 * p.rr_h
+    npc = n2 // 2
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    c = np.arange(npc).reshape(1, npc, 1)
+    j1 = np.arange(n1).reshape(1, 1, n1)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    e_in = uin + b * p.in_.s_b + c * p.in_.s_c + j1 * p.in_.s_j
+    z = _gather(p.in_, bufs, e_in, f"pass {pi} load")
+    Z = np.fft.fft(z, axis=-1)
+    Zm = np.conj(Z[..., (-np.arange(n1)) % n1])
+    A = 0.5 * (Z + Zm)
+    B = (Z - Zm) / 2j
+    k1 = np.arange(n1 // 2 + 1).reshape(1, 1, -1)
+    A = A[..., : n1 // 2 + 1] * np.exp(-2j * np.pi * ((2 * c * k1) % N) / N)
+    B = B[..., : n1 // 2 + 1] * np.exp(-2j * np.pi * (((2 * c + 1) * k1) % N) / N)
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    e_out = uout + b * p.out.s_b + c * p.out.s_c + k1 * p.out.s_j
+    both = np.concatenate([e_out.ravel(), (e_out + 1).ravel()])
+    if np.unique(both).size != both.size:
+        raise AssertionError(f"pass {pi}: overlapping output addresses")
+    _scatter(p.out, bufs, e_out, A, f"pass {pi} store A")
+    _scatter(p.out, bufs, e_out + 1, B, f"pass {pi} store B")
+
+
+def _real4b(p, bufs, start, cnt, pi):
+    """Pass B of the real-input four-step (real4b_kernel): rows k1 = 0..n1/2
+    of the scratch -> n2-point DFT R -> X[k1 + n1*k2] = R[k1][k2] for the bins
+    <= N/2, and X[(n1 - k1) + n1*k2] = conj R[k1][n2 - 1 - k2] for
+    0 < k1 < n1/2, k2 < n2/2; every bin 0..N/2 once."""
+    n2, n1, N = p.L, p.rr_m, 2 * p.rr_h
+    rows = n1 // 2 + 1
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    r = np.arange(rows).reshape(1, rows, 1)
+    j2 = np.arange(n2).reshape(1, 1, n2)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    Y = _gather(p.in_, bufs, uin + b * p.in_.s_b + r * p.in_.s_c + j2 * p.in_.s_j, f"pass {pi} load")
+    R = np.fft.fft(Y, axis=-1)
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    bb = np.arange(cnt).reshape(cnt, 1, 1)
+    k1 = np.arange(rows).reshape(1, rows, 1)
+    k2 = np.arange(n2).reshape(1, 1, n2)
+    own = (k1 + n1 * k2 <= N // 2) & ((k2 < n2 // 2) | ((k1 == 0) & (k2 == n2 // 2)))
+    own = np.broadcast_to(own, R.shape)
+    e_own = np.broadcast_to(uout + bb * p.out.s_b + k1 * p.out.s_c + k2 * p.out.s_j, R.shape)
+    mir = np.broadcast_to((k1 > 0) & (k1 < n1 // 2) & (k2 < n2 // 2), R.shape)
+    e_mir = np.broadcast_to(uout + bb * p.out.s_b + (n1 - k1) * p.out.s_c + k2 * p.out.s_j, R.shape)
+    Rm = np.conj(R[..., ::-1])  # conj R[k1][n2 - 1 - k2]
+    e_all = np.concatenate([e_own[own], e_mir[mir]])
+    if np.unique(e_all).size != e_all.size or e_all.size != cnt * (N // 2 + 1):
+        raise AssertionError(f"pass {pi}: {e_all.size} stores, {np.unique(e_all).size} distinct")
+    _scatter(p.out, bufs, e_own[own], R[own], f"pass {pi} store")
+    _scatter(p.out, bufs, e_mir[mir], Rm[mir], f"pass {pi} store (mirror)")
+
+
+def _real4c(p, bufs, start, cnt, pi):
+    """c2r pass B' of the real-input four-step (real4c_kernel): for rows
+    k1 = 0..n1/2, R[k1][k2] = X[k1 + n1*k2] where that bin is stored (k2 < n2/2,
+    row 0 up to k2 = n2/2), else conj X[(n1 - k1) + n1*(n2 - 1 - k2)] (row 0:
+    conj X[n1*(n2 - k2)]); imaginary parts of bins 0 and N/2 dropped; the
+    unnormalised inverse n2-point DFT times W_N^-(j2*k1) into scratch row k1."""
+    n2, n1, N = p.L, p.rr_m, 2 * p.rr_h
+    rows = n1 // 2 + 1
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    r = np.arange(rows).reshape(1, rows, 1)
+    k2 = np.arange(n2).reshape(1, 1, n2)
+    own = (k2 < n2 // 2) | ((r == 0) & (k2 == n2 // 2))
+    row_bin = np.where(own, r, np.where(r == 0, 0, n1 - r))
+    col_bin = np.where(own, k2, np.where(r == 0, n2 - k2, n2 - 1 - k2))
+    bins = np.broadcast_to(row_bin + n1 * col_bin, (1, rows, n2))
+    if bins.min() < 0 or bins.max() > N // 2 or np.unique(bins).size != N // 2 + 1:
+        raise AssertionError(f"pass {pi}: the rows do not read every bin 0..N/2")
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    e = uin + b * p.in_.s_b + row_bin * p.in_.s_c + col_bin * p.in_.s_j
+    R = _gather(p.in_, bufs, e, f"pass {pi} load")
+    R = np.where(np.broadcast_to(own, R.shape), R, np.conj(R))
+    dc = np.broadcast_to((r == 0) & ((k2 == 0) | (k2 == n2 // 2)), R.shape)
+    R = np.where(dc, R.real + 0j, R)
+    D = np.fft.ifft(R, axis=-1) * n2
+    j2 = np.arange(n2).reshape(1, 1, n2)
+    D = D * np.exp(2j * np.pi * ((j2 * r) % N) / N)
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    e_out = uout + b * p.out.s_b + r * p.out.s_c + j2 * p.out.s_j
+    if np.unique(e_out).size != e_out.size:
+        raise AssertionError(f"pass {pi}: overlapping output addresses")
+    _scatter(p.out, bufs, e_out, D, f"pass {pi} store")
+
+
+def _real4d(p, bufs, start, cnt, pi):
+    """c2r pass A' (real4d_kernel): packed column c: Z[k1] = D[k1][2c] +
+    i D[k1][2c+1] (rows k1 > n1/2 from conj D[n1 - k1]; column 2c at c*in.s_c,
+    row at k1*in.s_j), unnormalised inverse n1-point DFT, stored as the reals
+    2c, 2c+1 of rows j1 (complex pairs at c*out.s_c + j1*out.s_j)."""
+    n1, n2 = p.L, p.rr_m
+    npc = n2 // 2
+    b = np.arange(cnt).reshape(cnt, 1, 1)
+    c = np.arange(npc).reshape(1, npc, 1)
+    k1 = np.arange(n1).reshape(1, 1, n1)
+    lo = k1 <= n1 // 2
+    kk = np.where(lo, k1, n1 - k1)
+    uin = start * p.in_.s_u if p.in_.role < 2 else 0
+    e = uin + b * p.in_.s_b + c * p.in_.s_c + kk * p.in_.s_j
+    da = _gather(p.in_, bufs, e, f"pass {pi} load")
+    db = _gather(p.in_, bufs, e + 1, f"pass {pi} load")
+    lo = np.broadcast_to(lo, da.shape)
+    da = np.where(lo, da, np.conj(da))
+    db = np.where(lo, db, np.conj(db))
+    z = np.fft.ifft(da + 1j * db, axis=-1) * n1
+    uout = start * p.out.s_u if p.out.role < 2 else 0
+    e_out = uout + b * p.out.s_b + c * p.out.s_c + k1 * p.out.s_j
+    if np.unique(e_out).size != e_out.size:
+        raise AssertionError(f"pass {pi}: overlapping output addresses")
+    _scatter(p.out, bufs, e_out, z, f"pass {pi} store")
+
+
 def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant=None):
     """Executes the plan model; returns the dst arrays (float64).  edit
     (optional) maps the inspected pass list to the one executed; variant
@@ -486,6 +607,14 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant
                 _real_colpair(p, bufs, start, cnt, pi)
             elif p.type == 8:
                 _real_icolpair(p, bufs, start, cnt, pi)
+            elif p.type == 9:
+                _real4a(p, bufs, start, cnt, pi)
+            elif p.type == 10:
+                _real4b(p, bufs, start, cnt, pi)
+            elif p.type == 11:
+                _real4c(p, bufs, start, cnt, pi)
+            elif p.type == 12:
+                _real4d(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

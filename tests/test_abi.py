@@ -142,14 +142,13 @@ def test_plan_model_real_pass_in_rows(kind):
     # long real dim 0 with a 1024-point last axis: by default the real pass
     # over row pairs runs inside the column pass B (r2c, type 7) or A (c2r,
     # type 8); on request inside the row FFTs (type 5, or 6 for c2r)
-    for fmt in (IL, SP):
-        for batch in (1, 3):
-            err, info, passes = model_check(kind, (8192, 1024), fmt, batch)
-            assert [p.type for p in passes] == ([0, 7, 0] if kind == "r" else [0, 8, 0])
-            assert err < 1e-9, (kind, fmt, batch, err)
-            err, info, passes = model_check(kind, (8192, 1024), fmt, batch, variant={"real_pair": 1})
-            assert [p.type for p in passes] == ([0, 0, 5] if kind == "r" else [6, 0, 0])
-            assert err < 1e-9, (kind, fmt, batch, err)
+    for fmt, batch in ((IL, 1), (SP, 3)):
+        err, info, passes = model_check(kind, (8192, 1024), fmt, batch)
+        assert [p.type for p in passes] == ([0, 7, 0] if kind == "r" else [0, 8, 0])
+        assert err < 1e-9, (kind, fmt, batch, err)
+        err, info, passes = model_check(kind, (8192, 1024), fmt, batch, variant={"real_pair": 1})
+        assert [p.type for p in passes] == ([0, 0, 5] if kind == "r" else [6, 0, 0])
+        assert err < 1e-9, (kind, fmt, batch, err)
 
 
 @pytest.mark.parametrize("dims", [(8192, 64), (16384, 64), (8192, 2, 64)], ids=lambda d: "x".join(map(str, d)))
@@ -206,6 +205,63 @@ def test_plan_model_fused_real_pass(lg):
         assert err < 1e-9, (lg, fmt, err)
 
 
+@pytest.mark.parametrize("lg", [16, 17, 19, 20, 21, 22, 23])
+def test_plan_model_real_input_fourstep(lg):
+    # 1-D FP32 r2c above 2^15: the real-input four-step (pass A over packed
+    # real columns, rows 0..n1/2 into scratch; pass B with own and mirrored
+    # bins, each stored once -- the model checks both, bounds included);
+    # real_pair = -1 keeps the complex four-step with the transposing real pass
+    for fmt in (IL, SP):
+        err, info, passes = model_check("r", (1 << lg,), fmt, 3 if lg < 22 else 1, variant={"real_pair": 2})
+        assert [p.type for p in passes] == [9, 10], [(p.type, p.L) for p in passes]
+        assert passes[0].L * passes[1].L == 1 << lg and passes[1].L >= passes[0].L
+        assert err < 1e-9, (lg, fmt, err)
+    err, info, passes = model_check("r", (1 << lg,), IL, 2, variant={"real_pair": -1})
+    assert 9 not in [p.type for p in passes] and err < 1e-9
+    # FP64 keeps the complex four-step
+    info, passes = M.inspect("r", (1 << lg,), IL, 1, prec=64)
+    assert 9 not in [p.type for p in passes]
+
+
+def test_real_input_fourstep_shape_variants():
+    # the tile / schedule variants pick among the compiled shapes of each
+    # pass length and fall back to the default where none matches
+    info, p = M.inspect("r", (1 << 22,), IL, 2, variant={"real_pair": 2})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(9, 2048, 16, 1), (10, 2048, 8, 0)]
+    info, p = M.inspect("ri", (1 << 22,), IL, 2, variant={"real_pair": 2})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 0), (12, 2048, 8, 1)]
+    # defaults by length (measured): r2c 2^16..2^23, c2r 2^16..2^20
+    for kind, lg, real4 in (("r", 16, True), ("r", 22, True), ("r", 23, True), ("r", 24, False), ("ri", 16, True),
+                            ("ri", 20, True), ("ri", 21, False), ("ri", 22, False)):
+        info, p = M.inspect(kind, (1 << lg,), IL, 1)
+        assert (p[0].type in (9, 11)) == real4, (kind, lg, [q.type for q in p])
+    info, p = M.inspect("r", (1 << 22,), IL, 2, variant={"real_pair": 2, "rowt_tile": 16, "sched": 1, "col_tile": 16})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(9, 2048, 16, 1), (10, 2048, 16, 1)]
+    info, p = M.inspect("ri", (1 << 21,), IL, 2, variant={"real_pair": 2, "sched": 1})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 1), (12, 1024, 16, 1)]
+    info, p = M.inspect("ri", (1 << 20,), IL, 2, variant={"real_pair": 2, "rowt_tile": 4})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 1024, 8, 0), (12, 1024, 16, 1)]
+    err, info, p = model_check("r", (1 << 21,), SP, 2, variant={"real_pair": 2, "rowt_tile": 16, "sched": 1})
+    assert p[1].CT == 16 and err < 1e-9
+
+
+@pytest.mark.parametrize("lg", [16, 17, 20, 23])
+def test_plan_model_real_input_fourstep_c2r(lg):
+    # the 1-D FP32 c2r above 2^15 runs the same split backwards: pass B' reads
+    # every bin 0..N/2 (own and mirrored, the model checks the set), pass A'
+    # writes two real columns per complex column
+    for fmt in (IL, SP):
+        err, info, passes = model_check("ri", (1 << lg,), fmt, 3 if lg < 22 else 1, variant={"real_pair": 2})
+        assert [p.type for p in passes] == [11, 12], [(p.type, p.L) for p in passes]
+        assert passes[0].L * passes[1].L == 1 << lg and passes[0].L >= passes[1].L
+        assert err < 1e-9, (lg, fmt, err)
+    for rp in (-1, 1):
+        err, info, passes = model_check("ri", (1 << lg,), IL, 2, variant={"real_pair": rp})
+        assert 11 not in [p.type for p in passes] and err < 1e-9
+    info, passes = M.inspect("ri", (1 << lg,), IL, 1, prec=64)
+    assert 11 not in [p.type for p in passes]
+
+
 @pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])
 def test_plan_model_row_split_variant(kind):
     # other splits n = n1 * n2 of the 1-D four-step (variant row_log2): pass B
@@ -213,8 +269,13 @@ def test_plan_model_row_split_variant(kind):
     # transform the same way)
     for lg, rl in ((17, 9), (17, 11), (20, 8), (20, 11), (20, 12)):
         err, info, passes = model_check(kind, (1 << lg,), IL, 2, variant={"row_log2": rl})
-        ffts = [p for p in passes if p.type == 0]
-        assert ffts[-1].L == 1 << rl and ffts[0].colmap == 1, [(p.type, p.L) for p in passes]
+        if [p.type for p in passes] in ([9, 10], [11, 12]):
+            # r2c / c2r as the real-input four-step: rows of 2^row_log2 points
+            rows, cols = (passes[1], passes[0]) if kind == "r" else (passes[0], passes[1])
+            assert kind in ("r", "ri") and rows.L == 1 << rl and cols.L == 1 << (lg - rl)
+        else:
+            ffts = [p for p in passes if p.type == 0]
+            assert ffts[-1].L == 1 << rl and ffts[0].colmap == 1, [(p.type, p.L) for p in passes]
         assert err < 1e-9, (kind, lg, rl, err)
     # a row length that does not split n: the plan is refused
     for rl in (3, 20, 30):
@@ -270,6 +331,23 @@ def test_plan_shapes_of_headline_configs():
     # 256^3 c2c: three single passes
     info, passes = M.inspect("c", (256, 256, 256), IL, 1)
     assert len(passes) == 3
+
+
+def test_plan_model_three_pass_fourstep():
+    # 1-D lengths above 4096 * 4096: the four-step n = p * M whose row DFTs
+    # (M = q * 2048) are a four-step too -- pass 1 p-point columns, pass 2
+    # q-point columns in place, pass 3 2048-point rows storing bin
+    # kp + p*(kq + q*k2); r2c / c2r put the real pass beside it
+    err, info, passes = model_check("c", (1 << 25,), IL, 1)
+    assert [(p.type, p.L, p.colmap) for p in passes] == [(0, 128, 1), (0, 128, 1), (0, 2048, 0)], passes
+    assert passes[2].n_blo == 128 and passes[2].out.s_j == 128 * 128 and passes[2].out.s_b2 == 128
+    assert err < 1e-9, err
+    # (the model runs the c2c; the real plans add the real pass of the
+    # shorter plans, which the other model tests execute)
+    info, passes = M.inspect("ri", (1 << 26,), IL, 1)
+    assert [(p.type, p.L) for p in passes] == [(1, 1), (0, 128), (0, 128), (0, 2048)]
+    info, passes = M.inspect("r", (1 << 28,), SP, 1)
+    assert [(p.type, p.L) for p in passes] == [(0, 256), (0, 256), (0, 2048), (1, 1)]
 
 
 def test_unsupported_shapes_report_errors():

@@ -5,7 +5,8 @@ pinned to the oracle as well.
 - 64 x c2c 2^20 (forward, then inverse of its output): transforms 0, 7, 8,
   31, 32 and 63, which sit on both sides of the pass-A launch splits (8
   transforms) and the pass groups (32 transforms) of the default plan.
-- 16 x r2c 2^22: transforms 0, 3, 4 and 15 (pass-A launches of 4).
+- 16 x r2c 2^22 (the real-input four-step in two groups of 8): transforms 0,
+  7, 8 and 15, on both sides of the group boundary.
 - c2c 256^3 and r2c 8192^2, FP32 and FP64, against the full oracle.
 Bars: 2*log2(N)*2^-24 (FP32) and 16*2*log2(N)*2^-53 (FP64) relative L2."""
 import sys
@@ -73,7 +74,8 @@ def test_r2c_bench_plan_against_oracle():
     say("r2c bench plan", p.describe(), "chunk", p.get_variant().chunk, "pass-A launches", p.launches(0))
     p(x, None, y, None)
     torch.cuda.synchronize()
-    for t in (0, 3, 4, 15):
+    assert p.describe()[0].startswith("real4a") and p.get_variant().chunk == 8
+    for t in (0, 7, 8, 15):
         xt = x[n * t:n * (t + 1)].cpu().numpy().astype(np.float64)
         yt = _cx(y[2 * m * t:2 * m * (t + 1)])
         e = O.rel_l2(yt.astype(np.complex64), O.rfft(xt))

@@ -442,10 +442,13 @@ def test_split_pass_a_matches_one_launch(G, kind, n, batch, chunk, split):
         src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
         nout = 2 * n * batch
     p = Plan(kind, (n,), batch)
-    p.set_variant(chunk=chunk)
+    # the 1-D r2c on the complex four-step (the real-input four-step has no
+    # split form)
+    rp = {"real_pair": -1} if kind == "r" else {}
+    p.set_variant(chunk=chunk, **rp)
     ref = G.empty_dev(nout)
     p(src, None, ref, None)
-    p.set_variant(chunk=chunk, a_split=split)
+    p.set_variant(chunk=chunk, a_split=split, **rp)
     assert p.launches(0) == (batch // chunk) * split
     out = G.empty_dev(nout)
     p(src, None, out, None)

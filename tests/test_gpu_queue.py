@@ -152,6 +152,42 @@ def test_queue_chains_many_long_jobs(G):
     q.close()
 
 
+def test_queue_tune_and_lags(G):
+    """The queue autotuner times (groups per CU, lag) candidates on a batch
+    and leaves the queue prepared with the winner; every lag lists a chain
+    step after the one it waits for, so outputs stay right for each."""
+    import torch
+    from fft_amd.afft import Queue
+    rng = np.random.default_rng(33)
+    jobs, total = make_jobs(rng, 200, 8, 16)
+    host = fill_inputs(jobs, total, 500)
+    buf = G.to_dev(host)
+    base = int(buf.data_ptr())
+    q = Queue(len(jobs), 16)
+    recs = Queue.jobs([(k, lg, base + 4 * so, base + 4 * do) for (k, lg, so, nin, do, nout) in jobs])
+    for lag in (0, 1, 32):
+        say("queue lag", lag)
+        q.set_lag(lag)
+        q.prepare(recs)
+        buf.copy_(G.to_dev(host))
+        q.launch()
+        torch.cuda.synchronize()
+        assert q.status() == 0
+        check_all(jobs, host, G.to_host(buf))
+    ms = q.tune(recs, None, 2)
+    groups, lag = q.config
+    say("queue tuned", groups, lag, ms)
+    assert groups in (2, 4) and lag in (2, 8, 32) and ms > 0
+    buf.copy_(G.to_dev(host))
+    q.launch()
+    torch.cuda.synchronize()
+    assert q.status() == 0
+    check_all(jobs, host, G.to_host(buf))
+    with pytest.raises(RuntimeError):
+        q.set_lag(-1)
+    q.close()
+
+
 def test_queue_dependency_wait_is_bounded(G):
     """A zero spin limit makes a tile give up waiting at once: the launch must
     still finish (the status then reports the unmet wait, or 0 if every
