@@ -135,9 +135,10 @@ constexpr idx_t kColDirectMax = 2048;
 // has kernels for); longer ones as the complex four-step with a real pass.
 // Measured over 2^16..2^23 with 256 MiB of reals per batch: r2c 0.31 vs 0.38
 // ms (2^16), 0.34 vs 0.43 (2^20), 0.43 vs 0.45 (2^22), 0.47 vs 0.49 (2^23);
-// c2r 0.29 vs 0.33 (2^16), 0.35 vs 0.40 (2^20), but 0.41 vs 0.35 (2^21).
+// c2r 0.29 vs 0.35 (2^16), 0.32 vs 0.33 (2^20), 0.36 vs 0.40 (2^22), but
+// 0.52 vs 0.43 (2^23, whose 4096-point spectrum columns fit 4 per group).
 constexpr idx_t kReal4R2cMax = (idx_t)1 << 23;
-constexpr idx_t kReal4C2rMax = (idx_t)1 << 20;
+constexpr idx_t kReal4C2rMax = (idx_t)1 << 22;
 
 struct Variant {
   int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
@@ -815,10 +816,13 @@ struct Builder {
     idx_t n2 = P.var.row_log2 > 0 && P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : (idx_t)1 << (lg - lg / 2);
     if (n2 >= N) return false;
     const idx_t n1 = N / n2;
-    int cta = 0, sva = 0, ctb = 0, svb = 0;
+    int cta = 0, sva = 0, hxa = 0, ctb = 0, svb = 0, hxb = 0;
     const int want_sv = P.var.sched < 0 ? 0 : (P.var.sched > 0 ? 1 : -1);
-    if (!real4_col_shape((int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), false, P.var.col_tile, want_sv, cta, sva) ||
-        !real4_row_shape((int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.rowt_tile, want_sv, ctb, svb) ||
+    const int want_hx = P.var.half_lds < 0 ? 0 : (P.var.half_lds > 0 ? 1 : -1);
+    if (!real4_shape(0, (int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), P.var.col_tile, want_sv, want_hx, cta, sva,
+                     hxa) ||
+        !real4_shape(1, (int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.rowt_tile, want_sv, want_hx, ctb,
+                     svb, hxb) ||
         (n2 / 2) % cta != 0)
       return false;
     const int bits = (lg + 1) / 2;
@@ -850,7 +854,7 @@ struct Builder {
     a->rr_h = h;
     a->rr_m = n2;
     a->stage_tw = sa;
-    a->hx = 1;
+    a->hx = hxa;
     a->tw4_hi = tw4;
     a->tw4_lo = (const cx<T>*)tw4 + (N >> bits);
     a->tw4_bits = bits;
@@ -860,7 +864,7 @@ struct Builder {
     b->L = (int)n2;
     b->CT = ctb;
     b->sv = svb;
-    b->hx = 1;
+    b->hx = hxb;
     scratch_side(b->in, R_SCR);
     b->in.s_j = 1;
     b->in.s_c = n2;
@@ -877,9 +881,10 @@ struct Builder {
   }
 
   // 1-D FP32 c2r of h+1 bins to N = 2h reals, the same split run backwards:
-  // pass B' the inverse row DFTs of rows k1 <= n1/2 (each row read from the
-  // stored bins and the conjugates of the mirrored ones) into scratch, pass A'
-  // the inverse column DFTs of two real columns per complex column.
+  // pass B' the inverse DFTs of the spectrum's columns k1 <= n1/2 (bins
+  // k1 + n1*k2, past N/2 the conjugates of the mirrored column) into the
+  // scratch Dt[j2][k1], pass A' the inverse DFTs along the rows of Dt, two
+  // real columns per complex column, stored through LDS.
   bool c2r_real4(const Side& in, const Side& out, idx_t h) {
     const idx_t N = 2 * h;
     if (sizeof(T) != 4 || !(P.var.real_pair == 2 || (P.var.real_pair == 0 && N <= kReal4C2rMax))) return false;
@@ -887,64 +892,69 @@ struct Builder {
     idx_t n2 = P.var.row_log2 > 0 && P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : (idx_t)1 << (lg - lg / 2);
     if (n2 >= N) return false;
     const idx_t n1 = N / n2;
-    int cta = 0, sva = 0, ctb = 0, svb = 0;
+    int ctc = 0, svc = 0, hxc = 0, ctd = 0, svd = 0, hxd = 0;
     const int want_sv = P.var.sched < 0 ? 0 : (P.var.sched > 0 ? 1 : -1);
-    if (!real4_col_shape((int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), true, P.var.col_tile, want_sv, cta, sva) ||
-        !real4_row_shape((int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.rowt_tile, want_sv, ctb, svb) ||
-        (n2 / 2) % cta != 0)
+    const int want_hx = P.var.half_lds < 0 ? 0 : (P.var.half_lds > 0 ? 1 : -1);
+    if (!real4_shape(2, (int)sizeof(T), (int)std::min<idx_t>(n2, 1 << 30), P.var.col_tile, want_sv, want_hx, ctc, svc,
+                     hxc) ||
+        !real4_shape(3, (int)sizeof(T), (int)std::min<idx_t>(n1, 1 << 30), P.var.rowt_tile, want_sv, want_hx, ctd,
+                     svd, hxd) ||
+        (n2 / 2) % ctd != 0)
       return false;
     const int bits = (lg + 1) / 2;
     const void* tw4 = get_table<T>(1, N, bits);
-    const void* sa = get_table<T>(0, n1, sva);
-    const void* sb = get_table<T>(0, n2, svb);
-    if (!tw4 || !sa || !sb) {
+    const void* sc = get_table<T>(0, n2, svc);
+    const void* sd = get_table<T>(0, n1, svd);
+    if (!tw4 || !sc || !sd) {
       ok = false;
       return true;
     }
-    const idx_t rows = n1 / 2 + 1;
-    need_scratch(R_SCR, rows * n2);
-    PassRec* b = add();
-    if (!b) return true;
-    b->type = PT_REAL4C;
-    b->L = (int)n2;
-    b->CT = ctb;
-    b->sv = svb;
-    b->hx = 1;
-    b->in = in;  // bins 0..h
-    b->in.s_j = n1;
-    b->in.s_c = 1;
-    b->in.s_b = in.s_u;
-    scratch_side(b->out, R_SCR);
-    b->out.s_j = 1;
-    b->out.s_c = n2;
-    b->out.s_b = rows * n2;
-    b->n_col = rows;
-    b->rr_h = h;
-    b->rr_m = n1;
-    b->stage_tw = sb;
-    b->tw4_hi = tw4;
-    b->tw4_lo = (const cx<T>*)tw4 + (N >> bits);
-    b->tw4_bits = bits;
-    PassRec* a = add();
-    if (!a) return true;
-    a->type = PT_REAL4D;
-    a->L = (int)n1;
-    a->CT = cta;
-    a->colmap = 1;
-    a->hx = 1;
-    a->sv = sva;
-    scratch_side(a->in, R_SCR);
-    a->in.s_j = n2;
-    a->in.s_c = 2;
-    a->in.s_b = rows * n2;
-    a->out = out;  // N reals as h interleaved complex pairs
-    a->out.s_j = n2 / 2;
-    a->out.s_c = 1;
-    a->out.s_b = out.s_u;
-    a->n_col = n2 / 2;
-    a->rr_h = h;
-    a->rr_m = n2;
-    a->stage_tw = sa;
+    // rows of Dt padded to whole 128-byte lines
+    const idx_t rs = (n1 / 2 + 1 + 15) & ~(idx_t)15;
+    need_scratch(R_SCR, n2 * rs);
+    PassRec* c = add();
+    if (!c) return true;
+    c->type = PT_REAL4C;
+    c->L = (int)n2;
+    c->CT = ctc;
+    c->sv = svc;
+    c->colmap = 1;
+    c->hx = hxc;
+    c->in = in;  // bins 0..h: column k1 at k1, bin k2 of it at n1*k2
+    c->in.s_j = n1;
+    c->in.s_c = 1;
+    c->in.s_b = in.s_u;
+    scratch_side(c->out, R_SCR);
+    c->out.s_j = rs;
+    c->out.s_c = 1;
+    c->out.s_b = n2 * rs;
+    c->n_col = n1 / 2 + 1;
+    c->rr_h = h;
+    c->rr_m = n1;
+    c->stage_tw = sc;
+    c->tw4_hi = tw4;
+    c->tw4_lo = (const cx<T>*)tw4 + (N >> bits);
+    c->tw4_bits = bits;
+    PassRec* d = add();
+    if (!d) return true;
+    d->type = PT_REAL4D;
+    d->L = (int)n1;
+    d->CT = ctd;
+    d->sv = svd;
+    d->store_t = 1;
+    d->hx = hxd;
+    scratch_side(d->in, R_SCR);  // packed column c: rows 2c and 2c+1 of Dt
+    d->in.s_j = 1;
+    d->in.s_c = 2 * rs;
+    d->in.s_b = n2 * rs;
+    d->out = out;  // N reals as h interleaved complex pairs
+    d->out.s_j = n2 / 2;
+    d->out.s_c = 1;
+    d->out.s_b = out.s_u;
+    d->n_col = n2 / 2;
+    d->rr_h = h;
+    d->rr_m = n2;
+    d->stage_tw = sd;
     return true;
   }
 
@@ -1791,11 +1801,13 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     const bool c = r.type == PT_REAL4C;
     a.ct = r.CT;
     a.sv = r.sv;
+    a.hx = r.hx;
     a.n1 = c ? r.rr_m : r.L;
     a.n2 = c ? r.L : r.rr_m;
     a.nbatch = cnt;
     a.stage_tw = (const cx<T>*)r.stage_tw;
-    a.scr_bs = (a.n1 / 2 + 1) * a.n2;
+    a.rs = c ? r.out.s_j : r.in.s_c / 2;
+    a.scr_bs = a.n2 * a.rs;
     if (c) {
       a.in = ire;
       a.in_im = iim;
@@ -1822,6 +1834,7 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     const idx_t N = 2 * r.rr_h;
     a.ct = r.CT;
     a.sv = r.sv;
+    a.hx = r.hx;
     a.n1 = r.type == PT_REAL4A ? r.L : r.rr_m;
     a.n2 = r.type == PT_REAL4A ? r.rr_m : r.L;
     a.nbatch = cnt;
@@ -2524,16 +2537,16 @@ int describe_pass(const Plan& P, int i, char* buf, size_t len) {
     snprintf(buf, len, "real_icolpair L=%d CT=%d h2=%d h=%lld m=%lld in(role=%d il=%d)", r.L, r.CT, r.n_blo,
              (long long)r.rr_h, (long long)r.rr_m, r.in.role, r.in.il);
   else if (r.type == PT_REAL4A)
-    snprintf(buf, len, "real4a n1=%d CT=%d sv%d n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+    snprintf(buf, len, "real4a n1=%d CT=%d sv%d %s n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, r.hx ? "hx" : "lds", (long long)r.rr_m,
              r.in.role, r.out.role);
   else if (r.type == PT_REAL4B)
-    snprintf(buf, len, "real4b n2=%d CT=%d sv%d n1=%lld in(role=%d) out(role=%d il=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+    snprintf(buf, len, "real4b n2=%d CT=%d sv%d %s n1=%lld in(role=%d) out(role=%d il=%d)", r.L, r.CT, r.sv, r.hx ? "hx" : "lds", (long long)r.rr_m,
              r.in.role, r.out.role, r.out.il);
   else if (r.type == PT_REAL4C)
-    snprintf(buf, len, "real4c n2=%d CT=%d sv%d n1=%lld in(role=%d il=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+    snprintf(buf, len, "real4c n2=%d CT=%d sv%d %s n1=%lld in(role=%d il=%d) out(role=%d)", r.L, r.CT, r.sv, r.hx ? "hx" : "lds", (long long)r.rr_m,
              r.in.role, r.in.il, r.out.role);
   else if (r.type == PT_REAL4D)
-    snprintf(buf, len, "real4d n1=%d CT=%d sv%d n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, (long long)r.rr_m,
+    snprintf(buf, len, "real4d n1=%d CT=%d sv%d %s n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, r.hx ? "hx" : "lds", (long long)r.rr_m,
              r.in.role, r.out.role);
   else if (r.type == PT_REALCOLPAIR)
     snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,

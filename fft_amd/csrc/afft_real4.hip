@@ -1,5 +1,6 @@
-// afft_real4.hip -- 1-D r2c of a long real sequence as a real-input four-step
-// (N = n1 * n2 reals, two passes over half-size data; see afft_real4.h).
+// afft_real4.hip -- 1-D FP32 r2c and c2r of a long real sequence as a
+// real-input four-step (N = n1 * n2 reals, two passes over half-size data;
+// see afft_real4.h).
 //
 // afft computes its r2c as an N/2-point complex FFT of the packed sequence
 // x[2j] + i x[2j+1] followed by a real post-pass pairing bins k and N/2 - k
@@ -17,6 +18,7 @@
 //   pass B  runs the n2-point DFTs of those rows and stores, through LDS,
 //           X[k1 + n1*k2] for the bins up to N/2 and, for 0 < k1 < n1/2, the
 //           bins of row n1 - k1 as conj X[k1 + n1*(n2 - 1 - k2)].
+// The c2r runs the transposed arrangement backwards (passes B' and A' below).
 // Memory traffic: N reals in, N/2 complex through scratch, N/2 + 1 complex
 // out -- two round trips instead of three.
 #include "afft_real4.h"
@@ -223,12 +225,76 @@ __device__ __forceinline__ cx<T> r4_load_in(const Real4Args<T>& a, idx_t e) {
   return a.in_il ? ld_elem<T>(a.in, nullptr, 1, e, a.in_nt) : ld_elem<T>(a.in, a.in_im, 0, e, a.in_nt);
 }
 
-// c2r pass B': CT rows of L = n2 points per group (rows 0..n1/2), loaded
-// through LDS from the own and the mirrored bins, inverse DFT, W_N^-(j2*k1),
-// rows of D stored to scratch.
+// c2r pass B': CT consecutive columns k1 of the spectrum viewed as n2 rows of
+// n1 bins (bin k1 + n1*k2), columns k1 = 0..n1/2, L = n2 points each.  Bins
+// past N/2 come from the mirrored column: for k2 >= n2/2 (k1 > 0) conj X at
+// (n1 - k1) + n1*(n2 - 1 - k2), for column 0 conj X at n1*(n2 - k2); bins 0
+// and N/2 keep their real parts.  Own and mirrored loads are runs of CT
+// consecutive bins straight into registers.  Inverse DFT, times
+// W_N^-(j2*k1), stored as Dt[j2][k1] (rows of rs entries).
+template <typename T, int L, int CT, int SV, bool HX>
+__global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T, L, CT, SV, HX>::min_waves))
+    real4c_kernel(Real4Args<T> a) {
+  using S = R4AShape<T, L, CT, SV, HX>;
+  constexpr int E = S::E;
+  constexpr int TPT = S::TPT;
+  extern __shared__ __align__(16) unsigned char afgpu_smem[];
+  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
+  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
+  const int tid = threadIdx.x;
+  const int c = tid % CT, t = tid / CT;
+  const idx_t n1 = a.n1, half = n1 / 2;
+  const int col_tiles = (int)((half + 1 + CT - 1) / CT);
+  const unsigned bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const idx_t b = bid / col_tiles;
+  const idx_t k1 = (idx_t)(bid - b * col_tiles) * CT + c;
+  const bool live = k1 <= half;
+  const idx_t ib = b * a.in_bs;
+  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
+  auto latf = [&](int i) -> T& { return ldsf[lds_pad(i) * CT + c]; };
+
+  cx<T> v[E];
+  static_for<E>([&](auto mc) AFGPU_INLINE {
+    constexpr int m = decltype(mc)::value;
+    const int k2 = t + m * TPT;
+    cx<T> z = {T(0), T(0)};
+    if (live) {
+      if (m < E / 2 || (k1 == 0 && k2 == L / 2)) {
+        z = r4_load_in(a, ib + k1 + n1 * k2);
+        if (k1 == 0 && (k2 == 0 || k2 == L / 2)) z.y = T(0);  // bins 0 and N/2: real parts only
+      } else if (k1 == 0) {
+        z = conjg(r4_load_in(a, ib + n1 * (L - k2)));
+      } else {
+        z = conjg(r4_load_in(a, ib + (n1 - k1) + n1 * (L - 1 - k2)));
+      }
+    }
+    v[m] = swapri(z);  // inverse DFT as swap(DFT(swap(.)))
+  });
+  if constexpr (HX)
+    run_stages<T, L, SV, 0, true, CT>(v, t, latf, a.stage_tw);
+  else
+    run_stages<T, L, SV, 0, false, CT>(v, t, lat, a.stage_tw);
+  if (live) {
+    const unsigned mask = (1u << a.tw4_bits) - 1u;
+    cx<T>* dst = a.scr + b * a.scr_bs + k1;
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int j2 = t + m * TPT;
+      const cx<T> wt = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)j2 * (unsigned)k1);
+      dst[(idx_t)j2 * a.rs] = cmulc(swapri(v[m]), wt);
+    });
+  }
+}
+
+// c2r pass A': CT packed columns c of L = n1 points per group, each the
+// column spectrum of the real columns 2c and 2c+1 at once:
+// Z[k1] = Dt[2c][k1] + i Dt[2c+1][k1] (k1 > n1/2 from the conjugates at
+// n1 - k1), read along rows of Dt; inverse DFT; z[j1] = x[j1*n2 + 2c] +
+// i x[j1*n2 + 2c + 1] stored through LDS so that a group writes runs of CT
+// consecutive pairs of each output row.
 template <typename T, int L, int CT, int SV, bool HX>
 __global__ void __launch_bounds__((R4BShape<T, L, CT, SV, HX>::NT), (R4BShape<T, L, CT, SV, HX>::min_waves))
-    real4c_kernel(Real4Args<T> a) {
+    real4d_kernel(Real4Args<T> a) {
   using S = R4BShape<T, L, CT, SV, HX>;
   constexpr int E = S::E;
   constexpr int TPT = S::TPT;
@@ -238,127 +304,62 @@ __global__ void __launch_bounds__((R4BShape<T, L, CT, SV, HX>::NT), (R4BShape<T,
   T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
   const int tid = threadIdx.x;
   const int c = tid / TPT, t = tid % TPT;
-  const idx_t half = a.n1 / 2;
-  const idx_t nrows = half + 1;
-  const int row_tiles = (int)((nrows + CT - 1) / CT);
-  const unsigned bid = xcd_contiguous(blockIdx.x, gridDim.x);
-  const idx_t b = bid / row_tiles;
-  const idx_t row0 = (idx_t)(bid - b * row_tiles) * CT;
-  const idx_t row = row0 + c;
-  const idx_t n1 = a.n1;
-  const idx_t ib = b * a.in_bs;
-  auto lat = [&](int i) -> cx<T>& { return lds[c * LL + lds_pad(i)]; };
-  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
-
-  // load phase: lane c2 = tid % CT takes row row0 + c2, bins k2 = t2 + m*TPT
-  // (consecutive lanes read consecutive bins of one k2)
-  const int c2 = tid % CT, t2 = tid / CT;
-  const idx_t r2 = row0 + c2;
-  cx<T> w[E];
-  static_for<E>([&](auto mc) AFGPU_INLINE {
-    constexpr int m = decltype(mc)::value;
-    const int k2 = t2 + m * TPT;
-    cx<T> z = {T(0), T(0)};
-    if (r2 <= half) {
-      if (k2 < L / 2 || (r2 == 0 && k2 == L / 2)) {
-        z = r4_load_in(a, ib + r2 + n1 * k2);
-        if (r2 == 0 && (k2 == 0 || k2 == L / 2)) z.y = T(0);  // bins 0 and N/2: real parts only
-      } else if (r2 == 0) {
-        z = conjg(r4_load_in(a, ib + n1 * (L - k2)));
-      } else {
-        z = conjg(r4_load_in(a, ib + (n1 - r2) + n1 * (L - 1 - k2)));
-      }
-    }
-    w[m] = z;
-  });
-  cx<T> v[E];
-  if constexpr (!HX) {
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      lds[c2 * LL + lds_pad(t2 + m * TPT)] = w[m];
-    });
-    __syncthreads();
-    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value] = swapri(lat(t + decltype(mc)::value * TPT)); });
-    __syncthreads();
-  } else {
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      ldsf[c2 * LL + lds_pad(t2 + m * TPT)] = w[m].x;
-    });
-    __syncthreads();
-    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value].y = latf(t + decltype(mc)::value * TPT); });
-    __syncthreads();
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      ldsf[c2 * LL + lds_pad(t2 + m * TPT)] = w[m].y;
-    });
-    __syncthreads();
-    static_for<E>([&](auto mc) AFGPU_INLINE { v[decltype(mc)::value].x = latf(t + decltype(mc)::value * TPT); });
-    __syncthreads();
-  }
-  // inverse DFT as swap(DFT(swap(.))): v holds swap(R[row][k2])
-  if constexpr (HX)
-    run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
-  else
-    run_stages<T, L, SV, 0, false, 1>(v, t, lat, a.stage_tw);
-  if (row <= half) {
-    const unsigned mask = (1u << a.tw4_bits) - 1u;
-    cx<T>* dst = a.scr + b * a.scr_bs + row * a.n2;
-    static_for<E>([&](auto mc) AFGPU_INLINE {
-      constexpr int m = decltype(mc)::value;
-      const int j2 = t + m * TPT;
-      const cx<T> wt = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)j2 * (unsigned)row);
-      dst[j2] = cmulc(swapri(v[m]), wt);
-    });
-  }
-}
-
-// c2r pass A': CT packed columns of L = n1 points per group: the column
-// spectra of two real columns at once, inverse DFT, two columns of reals.
-template <typename T, int L, int CT, int SV, bool HX>
-__global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T, L, CT, SV, HX>::min_waves))
-    real4d_kernel(Real4Args<T> a) {
-  using S = R4AShape<T, L, CT, SV, HX>;
-  constexpr int E = S::E;
-  constexpr int TPT = S::TPT;
-  extern __shared__ __align__(16) unsigned char afgpu_smem[];
-  cx<T>* __restrict__ lds = reinterpret_cast<cx<T>*>(afgpu_smem);
-  T* __restrict__ ldsf = reinterpret_cast<T*>(afgpu_smem);
-  const int tid = threadIdx.x;
-  const int c = tid % CT, t = tid / CT;
   const idx_t npc = a.n2 / 2;
   const int col_tiles = (int)(npc / CT);
   const idx_t b = blockIdx.x / col_tiles;
-  const idx_t col = (idx_t)(blockIdx.x - b * col_tiles) * CT + c;
-  auto lat = [&](int i) -> cx<T>& { return lds[lds_pad(i) * CT + c]; };
-  auto latf = [&](int i) -> T& { return ldsf[lds_pad(i) * CT + c]; };
+  const idx_t c0 = (idx_t)(blockIdx.x - b * col_tiles) * CT;
+  auto lat = [&](int i) -> cx<T>& { return lds[c * LL + lds_pad(i)]; };
+  auto latf = [&](int i) -> T& { return ldsf[c * LL + lds_pad(i)]; };
 
-  // Z[k1] = D[k1][2col] + i D[k1][2col + 1]; rows k1 > n1/2 from conj D[n1 - k1]
-  const cx<T>* src = a.scr + b * a.scr_bs + 2 * col;
+  const cx<T>* ra = a.scr + b * a.scr_bs + 2 * (c0 + c) * a.rs;
+  const cx<T>* rb = ra + a.rs;
   cx<T> v[E];
   static_for<E>([&](auto mc) AFGPU_INLINE {
     constexpr int m = decltype(mc)::value;
     const int k1 = t + m * TPT;
     const bool lo = k1 <= L / 2;
-    const cx<T>* row = src + (idx_t)(lo ? k1 : L - k1) * a.n2;
-    cx<T> da = row[0], db = row[1];
+    const int kk = lo ? k1 : L - k1;
+    cx<T> da = ra[kk], db = rb[kk];
     if (!lo) {
       da = conjg(da);
       db = conjg(db);
     }
-    // swap(da + i db) for the inverse DFT
-    v[m] = cx<T>{da.y + db.x, da.x - db.y};
+    v[m] = cx<T>{da.y + db.x, da.x - db.y};  // swap(da + i db)
   });
   if constexpr (HX)
-    run_stages<T, L, SV, 0, true, CT>(v, t, latf, a.stage_tw);
+    run_stages<T, L, SV, 0, true, 1>(v, t, latf, a.stage_tw);
   else
-    run_stages<T, L, SV, 0, false, CT>(v, t, lat, a.stage_tw);
-  // z[j1] = swap(v) = x[j1*n2 + 2col] + i x[j1*n2 + 2col + 1]
-  T* dst = a.out_re + b * a.out_bs + 2 * col;
-  static_for<E>([&](auto mc) AFGPU_INLINE {
-    constexpr int m = decltype(mc)::value;
-    st_elem<T>(dst, nullptr, 1, (idx_t)(t + m * TPT) * npc, swapri(v[m]), a.out_nt);
-  });
+    run_stages<T, L, SV, 0, false, 1>(v, t, lat, a.stage_tw);
+
+  // v[m] = swap(z[j1]), j1 = t + m*TPT.  Store phase: lane c2 = tid % CT
+  // takes packed column c0 + c2, j1 = t2 + m*TPT.
+  const int c2 = tid % CT, t2 = tid / CT;
+  T* dst = a.out_re + b * a.out_bs + 2 * (c0 + c2);
+  if constexpr (!HX) {
+    static_for<E>([&](auto mc) AFGPU_INLINE { lat(t + decltype(mc)::value * TPT) = swapri(v[decltype(mc)::value]); });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int j1 = t2 + m * TPT;
+      st_elem<T>(dst, nullptr, 1, (idx_t)j1 * npc, lds[c2 * LL + lds_pad(j1)], a.out_nt);
+    });
+  } else {
+    T xr[E];
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].y; });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      xr[m] = ldsf[c2 * LL + lds_pad(t2 + m * TPT)];
+    });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE { latf(t + decltype(mc)::value * TPT) = v[decltype(mc)::value].x; });
+    __syncthreads();
+    static_for<E>([&](auto mc) AFGPU_INLINE {
+      constexpr int m = decltype(mc)::value;
+      const int j1 = t2 + m * TPT;
+      st_elem<T>(dst, nullptr, 1, (idx_t)j1 * npc, cx<T>{xr[m], ldsf[c2 * LL + lds_pad(j1)]}, a.out_nt);
+    });
+  }
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -405,7 +406,7 @@ static hipError_t launch_r4b(const Real4Args<T>& a, hipStream_t st) {
 
 template <typename T, int L, int CT, int SV, bool HX>
 static hipError_t launch_r4c(const Real4Args<T>& a, hipStream_t st) {
-  using S = R4BShape<T, L, CT, SV, HX>;
+  using S = R4AShape<T, L, CT, SV, HX>;
   auto k = real4c_kernel<T, L, CT, SV, HX>;
   if (S::lds > 65536) {
     static bool attr = false;
@@ -415,7 +416,7 @@ static hipError_t launch_r4c(const Real4Args<T>& a, hipStream_t st) {
       attr = true;
     }
   }
-  if (a.n2 != L) return hipErrorInvalidValue;
+  if (a.n2 != L || a.rs < a.n1 / 2 + 1) return hipErrorInvalidValue;
   const idx_t blocks = a.nbatch * ((a.n1 / 2 + 1 + CT - 1) / CT);
   if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
@@ -425,7 +426,7 @@ static hipError_t launch_r4c(const Real4Args<T>& a, hipStream_t st) {
 
 template <typename T, int L, int CT, int SV, bool HX>
 static hipError_t launch_r4d(const Real4Args<T>& a, hipStream_t st) {
-  using S = R4AShape<T, L, CT, SV, HX>;
+  using S = R4BShape<T, L, CT, SV, HX>;
   auto k = real4d_kernel<T, L, CT, SV, HX>;
   if (S::lds > 65536) {
     static bool attr = false;
@@ -436,7 +437,7 @@ static hipError_t launch_r4d(const Real4Args<T>& a, hipStream_t st) {
     }
   }
   const idx_t npc = a.n2 / 2;
-  if (npc % CT != 0 || a.n1 != L) return hipErrorInvalidValue;
+  if (npc % CT != 0 || a.n1 != L || a.rs < a.n1 / 2 + 1) return hipErrorInvalidValue;
   const idx_t blocks = a.nbatch * (npc / CT);
   if (blocks <= 0) return hipSuccess;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
@@ -445,91 +446,107 @@ static hipError_t launch_r4d(const Real4Args<T>& a, hipStream_t st) {
 }
 
 // Compiled shapes (length, tile, stage schedule), the default of each length
-// first.  Column passes (A, A'): packed columns per group -- 16 read 128-byte
-// row segments of the reals and write 256-byte ones.  Row passes (B, B'): rows
-// per group.  All use the half exchange, so two groups of <= 80 KiB share a CU.
+// first.  Column-mapped passes (A over n1-point columns of the reals, C over
+// n2-point columns of the spectrum): tiles of CT columns, 16 giving 128-byte
+// runs.  Row-mapped passes with a transposed store (B over n2-point rows, D
+// over n1-point rows of the c2r scratch): tiles of CT rows.  All use the half
+// exchange, so two groups of <= 80 KiB share a CU.
 struct R4Shape {
-  int L, CT, SV;
+  int L, CT, SV, HX;
 };
 // 2048-point columns: 16 per group for the r2c pass A (one group per CU,
-// 128-byte segments: 16 x r2c 2^22 0.469 -> 0.427 ms), 8 for the c2r pass A'
-// (two groups per CU: 0.122 against 0.155 ms for 16 x 2^22)
-constexpr R4Shape kR4ColA[] = {{128, 16, 0}, {256, 16, 0}, {512, 16, 1}, {1024, 16, 1}, {2048, 16, 1}, {2048, 8, 1}};
-constexpr R4Shape kR4ColD[] = {{128, 16, 0}, {256, 16, 0}, {512, 16, 1}, {1024, 16, 1}, {2048, 8, 1}, {2048, 16, 1}};
-constexpr R4Shape kR4Row[] = {{256, 16, 0}, {512, 16, 0}, {1024, 8, 0},  {1024, 16, 1}, {2048, 8, 0},
-                              {2048, 8, 1}, {2048, 16, 1}, {4096, 4, 0}};
+// 128-byte segments: 16 x r2c 2^22 0.469 -> 0.427 ms)
+#define AFGPU_R4A_SHAPES(X) X(128, 16, 0, 1) X(256, 16, 0, 1) X(512, 16, 1, 1) X(1024, 16, 1, 1) X(2048, 16, 1, 1) X(2048, 8, 1, 1)
+#define AFGPU_R4B_SHAPES(X)                                                                                 \
+  X(256, 16, 0, 1) X(512, 16, 0, 1) X(1024, 8, 0, 1) X(1024, 16, 1, 1) X(1024, 16, 0, 0) X(2048, 8, 0, 1)  \
+  X(2048, 8, 1, 1) X(2048, 16, 1, 1) X(2048, 8, 0, 0) X(4096, 4, 0, 1)
+// c2r spectrum pass: 8 columns per group at 1024 and 2048 points (64 x 2^20
+// 0.237 -> 0.207 ms, 16 x 2^22 0.254 -> 0.228 ms against 16 columns)
+#define AFGPU_R4C_SHAPES(X)                                                                                 \
+  X(256, 16, 0, 1) X(512, 16, 1, 1) X(1024, 8, 1, 1) X(1024, 16, 1, 1) X(1024, 16, 0, 0) X(2048, 8, 1, 1)   \
+  X(2048, 16, 1, 1) X(2048, 8, 0, 0) X(4096, 4, 0, 1)
+#define AFGPU_R4D_SHAPES(X)                                                                                 \
+  X(128, 16, 0, 1) X(256, 16, 0, 1) X(512, 16, 0, 1) X(1024, 8, 0, 1) X(1024, 16, 0, 0) X(1024, 16, 1, 1)  \
+  X(2048, 8, 0, 0) X(2048, 8, 1, 1) X(2048, 16, 1, 1)
+#define AFGPU_R4_ENTRY(LL, CTT, SVV, HXX) {LL, CTT, SVV, HXX},
+constexpr R4Shape kR4A[] = {AFGPU_R4A_SHAPES(AFGPU_R4_ENTRY)};
+constexpr R4Shape kR4B[] = {AFGPU_R4B_SHAPES(AFGPU_R4_ENTRY)};
+constexpr R4Shape kR4C[] = {AFGPU_R4C_SHAPES(AFGPU_R4_ENTRY)};
+constexpr R4Shape kR4D[] = {AFGPU_R4D_SHAPES(AFGPU_R4_ENTRY)};
+#undef AFGPU_R4_ENTRY
 
 template <size_t N>
-static bool r4_pick(const R4Shape (&tab)[N], int L, int want_ct, int want_sv, int& ct, int& sv) {
+static bool r4_pick(const R4Shape (&tab)[N], int L, int want_ct, int want_sv, int want_hx, int& ct, int& sv,
+                    int& hx) {
   const R4Shape* first = nullptr;
   for (const R4Shape& s : tab) {
     if (s.L != L) continue;
     if (!first) first = &s;
-    if ((want_ct <= 0 || s.CT == want_ct) && (want_sv < 0 || s.SV == want_sv)) {
+    if ((want_ct <= 0 || s.CT == want_ct) && (want_sv < 0 || s.SV == want_sv) && (want_hx < 0 || s.HX == want_hx)) {
       ct = s.CT;
       sv = s.SV;
+      hx = s.HX;
       return true;
     }
   }
   if (!first) return false;
   ct = first->CT;
   sv = first->SV;
+  hx = first->HX;
   return true;
 }
 
-bool real4_col_shape(int elem_bytes, int n1, bool inverse, int want_ct, int want_sv, int& ct, int& sv) {
-  return elem_bytes == 4 && (inverse ? r4_pick(kR4ColD, n1, want_ct, want_sv, ct, sv)
-                                     : r4_pick(kR4ColA, n1, want_ct, want_sv, ct, sv));
-}
-bool real4_row_shape(int elem_bytes, int n2, int want_ct, int want_sv, int& ct, int& sv) {
-  return elem_bytes == 4 && r4_pick(kR4Row, n2, want_ct, want_sv, ct, sv);
+bool real4_shape(int pass, int elem_bytes, int len, int want_ct, int want_sv, int want_hx, int& ct, int& sv,
+                 int& hx) {
+  if (elem_bytes != 4) return false;
+  switch (pass) {
+    case 0: return r4_pick(kR4A, len, want_ct, want_sv, want_hx, ct, sv, hx);
+    case 1: return r4_pick(kR4B, len, want_ct, want_sv, want_hx, ct, sv, hx);
+    case 2: return r4_pick(kR4C, len, want_ct, want_sv, want_hx, ct, sv, hx);
+    case 3: return r4_pick(kR4D, len, want_ct, want_sv, want_hx, ct, sv, hx);
+    default: return false;
+  }
 }
 
-#define AFGPU_R4_COL(F, LL, CTT, SVV) \
-  if (a.n1 == LL && a.ct == CTT && a.sv == SVV) return F<float, LL, CTT, SVV, true>(a, st);
-#define AFGPU_R4_ROW(F, LL, CTT, SVV) \
-  if (a.n2 == LL && a.ct == CTT && a.sv == SVV) return F<float, LL, CTT, SVV, true>(a, st);
-#define AFGPU_R4_COLS(F)            \
-  AFGPU_R4_COL(F, 128, 16, 0)       \
-  AFGPU_R4_COL(F, 256, 16, 0)       \
-  AFGPU_R4_COL(F, 512, 16, 1)       \
-  AFGPU_R4_COL(F, 1024, 16, 1)      \
-  AFGPU_R4_COL(F, 2048, 8, 1)       \
-  AFGPU_R4_COL(F, 2048, 16, 1)
-#define AFGPU_R4_ROWS(F)            \
-  AFGPU_R4_ROW(F, 256, 16, 0)       \
-  AFGPU_R4_ROW(F, 512, 16, 0)       \
-  AFGPU_R4_ROW(F, 1024, 8, 0)       \
-  AFGPU_R4_ROW(F, 1024, 16, 1)      \
-  AFGPU_R4_ROW(F, 2048, 8, 0)       \
-  AFGPU_R4_ROW(F, 2048, 8, 1)       \
-  AFGPU_R4_ROW(F, 2048, 16, 1)      \
-  AFGPU_R4_ROW(F, 4096, 4, 0)
+#define AFGPU_R4_N1(LL, CTT, SVV, HXX) \
+  if (a.n1 == LL && a.ct == CTT && a.sv == SVV && a.hx == HXX) return LAUNCH<float, LL, CTT, SVV, (HXX != 0)>(a, st);
+#define AFGPU_R4_N2(LL, CTT, SVV, HXX) \
+  if (a.n2 == LL && a.ct == CTT && a.sv == SVV && a.hx == HXX) return LAUNCH<float, LL, CTT, SVV, (HXX != 0)>(a, st);
 
 template <>
 hipError_t real4a<float>(const Real4Args<float>& a, hipStream_t st) {
-  AFGPU_R4_COLS(launch_r4a)
+#define LAUNCH launch_r4a
+  AFGPU_R4A_SHAPES(AFGPU_R4_N1)
+#undef LAUNCH
   return hipErrorInvalidValue;
 }
 template <>
 hipError_t real4b<float>(const Real4Args<float>& a, hipStream_t st) {
-  AFGPU_R4_ROWS(launch_r4b)
+#define LAUNCH launch_r4b
+  AFGPU_R4B_SHAPES(AFGPU_R4_N2)
+#undef LAUNCH
   return hipErrorInvalidValue;
 }
 template <>
 hipError_t real4c<float>(const Real4Args<float>& a, hipStream_t st) {
-  AFGPU_R4_ROWS(launch_r4c)
+#define LAUNCH launch_r4c
+  AFGPU_R4C_SHAPES(AFGPU_R4_N2)
+#undef LAUNCH
   return hipErrorInvalidValue;
 }
 template <>
 hipError_t real4d<float>(const Real4Args<float>& a, hipStream_t st) {
-  AFGPU_R4_COLS(launch_r4d)
+#define LAUNCH launch_r4d
+  AFGPU_R4D_SHAPES(AFGPU_R4_N1)
+#undef LAUNCH
   return hipErrorInvalidValue;
 }
-#undef AFGPU_R4_COLS
-#undef AFGPU_R4_ROWS
-#undef AFGPU_R4_COL
-#undef AFGPU_R4_ROW
+#undef AFGPU_R4_N1
+#undef AFGPU_R4_N2
+#undef AFGPU_R4A_SHAPES
+#undef AFGPU_R4B_SHAPES
+#undef AFGPU_R4C_SHAPES
+#undef AFGPU_R4D_SHAPES
 
 template <>
 hipError_t real4c<double>(const Real4Args<double>&, hipStream_t) {

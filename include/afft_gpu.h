@@ -138,7 +138,7 @@ typedef struct
                         2 runs the real-input four-step (two passes over N/2
                         complex: pass A reads two real columns as one complex
                         column, pass B stores each bin once; 2^16 <= N <= 2^23),
-                        the default for r2c up to 2^23 and c2r up to 2^20.
+                        the default for r2c up to 2^23 and c2r up to 2^22.
                         Longer ones (or -1) run the complex four-step of the
                         N/2 packed points with the real pass on its own: r2c
                         as a row pass into a second scratch and the transposing

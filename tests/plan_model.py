@@ -516,11 +516,13 @@ def _real4b(p, bufs, start, cnt, pi):
 
 
 def _real4c(p, bufs, start, cnt, pi):
-    """c2r pass B' of the real-input four-step (real4c_kernel): for rows
-    k1 = 0..n1/2, R[k1][k2] = X[k1 + n1*k2] where that bin is stored (k2 < n2/2,
-    row 0 up to k2 = n2/2), else conj X[(n1 - k1) + n1*(n2 - 1 - k2)] (row 0:
+    """c2r pass B' of the real-input four-step (real4c_kernel): for columns
+    k1 = 0..n1/2 of the spectrum (bin k1 + n1*k2 at k1*in.s_c + k2*in.s_j),
+    R[k1][k2] = X[k1 + n1*k2] where that bin is stored (k2 < n2/2, column 0
+    up to k2 = n2/2), else conj X[(n1 - k1) + n1*(n2 - 1 - k2)] (column 0:
     conj X[n1*(n2 - k2)]); imaginary parts of bins 0 and N/2 dropped; the
-    unnormalised inverse n2-point DFT times W_N^-(j2*k1) into scratch row k1."""
+    unnormalised inverse n2-point DFT times W_N^-(j2*k1) stored as Dt[j2][k1]
+    (at j2*out.s_j + k1*out.s_c)."""
     n2, n1, N = p.L, p.rr_m, 2 * p.rr_h
     rows = n1 // 2 + 1
     b = np.arange(cnt).reshape(cnt, 1, 1)
@@ -549,10 +551,11 @@ def _real4c(p, bufs, start, cnt, pi):
 
 
 def _real4d(p, bufs, start, cnt, pi):
-    """c2r pass A' (real4d_kernel): packed column c: Z[k1] = D[k1][2c] +
-    i D[k1][2c+1] (rows k1 > n1/2 from conj D[n1 - k1]; column 2c at c*in.s_c,
-    row at k1*in.s_j), unnormalised inverse n1-point DFT, stored as the reals
-    2c, 2c+1 of rows j1 (complex pairs at c*out.s_c + j1*out.s_j)."""
+    """c2r pass A' (real4d_kernel): packed column c: Z[k1] = Dt[2c][k1] +
+    i Dt[2c+1][k1] (k1 > n1/2 from the conjugates at n1 - k1; row 2c at
+    c*in.s_c, row 2c+1 in.s_c/2 further, k1 at k1*in.s_j), unnormalised
+    inverse n1-point DFT, stored as the reals 2c, 2c+1 of rows j1 (complex
+    pairs at c*out.s_c + j1*out.s_j)."""
     n1, n2 = p.L, p.rr_m
     npc = n2 // 2
     b = np.arange(cnt).reshape(cnt, 1, 1)
@@ -563,7 +566,7 @@ def _real4d(p, bufs, start, cnt, pi):
     uin = start * p.in_.s_u if p.in_.role < 2 else 0
     e = uin + b * p.in_.s_b + c * p.in_.s_c + kk * p.in_.s_j
     da = _gather(p.in_, bufs, e, f"pass {pi} load")
-    db = _gather(p.in_, bufs, e + 1, f"pass {pi} load")
+    db = _gather(p.in_, bufs, e + p.in_.s_c // 2, f"pass {pi} load")
     lo = np.broadcast_to(lo, da.shape)
     da = np.where(lo, da, np.conj(da))
     db = np.where(lo, db, np.conj(db))

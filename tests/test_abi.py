@@ -68,6 +68,16 @@ def test_memsize_conventions():
             assert ms(nd, dims, 1, 3) == 0  # afft_avx2: not provided here
             nd3, d3 = _lib.dims_array([12])
             assert ms(nd3, d3, 1, 0) == 0  # not a power of two
+    # memsize builds the plan host-only: 0 exactly where create would fail
+    # (a strided axis beyond two passes), > 0 for a 1-D length beyond 2^24
+    # (three passes)
+    nd4, d4 = _lib.dims_array([1 << 26, 2])
+    assert L.afft_32_c_memsize(nd4, d4, 1, 0) == 0
+    mem = ctypes.create_string_buffer(4096)
+    assert not L.afft_32_c_create(nd4, d4, 1, mem, 0)
+    nd5, d5 = _lib.dims_array([1 << 25])
+    for k in ("c", "ci", "r", "ri"):
+        assert getattr(L, f"afft_32_{k}_memsize")(nd5, d5, 1, 0) > 0
 
 
 def test_create_rejects_without_crashing():
@@ -229,10 +239,10 @@ def test_real_input_fourstep_shape_variants():
     info, p = M.inspect("r", (1 << 22,), IL, 2, variant={"real_pair": 2})
     assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(9, 2048, 16, 1), (10, 2048, 8, 0)]
     info, p = M.inspect("ri", (1 << 22,), IL, 2, variant={"real_pair": 2})
-    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 0), (12, 2048, 8, 1)]
-    # defaults by length (measured): r2c 2^16..2^23, c2r 2^16..2^20
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 1), (12, 2048, 8, 0)]
+    # defaults by length (measured): r2c 2^16..2^23, c2r 2^16..2^22
     for kind, lg, real4 in (("r", 16, True), ("r", 22, True), ("r", 23, True), ("r", 24, False), ("ri", 16, True),
-                            ("ri", 20, True), ("ri", 21, False), ("ri", 22, False)):
+                            ("ri", 20, True), ("ri", 22, True), ("ri", 23, False)):
         info, p = M.inspect(kind, (1 << lg,), IL, 1)
         assert (p[0].type in (9, 11)) == real4, (kind, lg, [q.type for q in p])
     info, p = M.inspect("r", (1 << 22,), IL, 2, variant={"real_pair": 2, "rowt_tile": 16, "sched": 1, "col_tile": 16})
@@ -240,7 +250,12 @@ def test_real_input_fourstep_shape_variants():
     info, p = M.inspect("ri", (1 << 21,), IL, 2, variant={"real_pair": 2, "sched": 1})
     assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 1), (12, 1024, 16, 1)]
     info, p = M.inspect("ri", (1 << 20,), IL, 2, variant={"real_pair": 2, "rowt_tile": 4})
-    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 1024, 8, 0), (12, 1024, 16, 1)]
+    assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 1024, 8, 1), (12, 1024, 8, 0)]
+    # the c2r pass A' without the half exchange (full complex entries in LDS)
+    info, p = M.inspect("ri", (1 << 22,), IL, 2, variant={"real_pair": 2, "half_lds": -1})
+    assert [(q.type, q.L, q.CT, q.sv) for q in p][1] == (12, 2048, 8, 0)
+    err, info, p = model_check("ri", (1 << 21,), SP, 2, variant={"real_pair": 2, "half_lds": -1, "rowt_tile": 16})
+    assert (p[1].L, p[1].CT) == (1024, 16) and err < 1e-9
     err, info, p = model_check("r", (1 << 21,), SP, 2, variant={"real_pair": 2, "rowt_tile": 16, "sched": 1})
     assert p[1].CT == 16 and err < 1e-9
 

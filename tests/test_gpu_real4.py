@@ -171,7 +171,9 @@ def test_real4_c2r_round_trip_grouping_and_other_forms():
 
 
 @pytest.mark.parametrize("var", [{"rowt_tile": 16}, {"sched": 1}, {"rowt_tile": 16, "sched": 1}, {"col_tile": 16},
-                                 {"col_tile": 16, "sched": 1}], ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()))
+                                 {"col_tile": 16, "sched": 1}, {"col_tile": 8}, {"half_lds": -1},
+                                 {"half_lds": -1, "rowt_tile": 16}],
+                         ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()))
 def test_real4_compiled_shapes_against_oracle(var):
     """Every compiled tile / schedule of the four passes (chosen through the
     variant fields) against the oracle, r2c and c2r, at the splits that use
