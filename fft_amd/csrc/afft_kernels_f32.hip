@@ -17,7 +17,6 @@
 #define AFGPU_COL_CT1024 16
 #define AFGPU_COL_CT2048 8
 #define AFGPU_COL_CT4096 4
-#define AFGPU_DUAL 1
 #define AFGPU_WIDE_COL1024 1
 #define AFGPU_C2R4 1
 #include "afft_variants.inc"

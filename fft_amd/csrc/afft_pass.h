@@ -686,26 +686,4 @@ __global__ void __launch_bounds__((PassShape<L, CT, SV>::NT), (pass_min_waves<T,
                                                          reinterpret_cast<cx<T>*>(afgpu_smem));
 }
 
-// Two independent passes in one launch: block b runs tile floor(b*nA/(nA+nB))
-// of pass A when that index steps at b, else the next unused tile of pass B, so
-// the two passes' tiles interleave evenly over the grid and share every CU.
-// Used for four-step plans over several groups: pass A of group k beside
-// pass B of group k-1 (separate scratch halves), which overlaps A's strided
-// reads and twiddle work with B's transposed writes.
-template <typename T, int LA, int CTA, int SVA, bool HXA, int LB, int CTB, int SVB, bool HXB>
-__global__ void __launch_bounds__((PassShape<LA, CTA, SVA>::NT), (PassShape<LA, CTA, SVA>::NT / 128))
-    fft_dual_kernel(PassParams<T> pa, PassParams<T> pb) {
-  static_assert(PassShape<LA, CTA, SVA>::NT == PassShape<LB, CTB, SVB>::NT, "both passes use one group size");
-  extern __shared__ __align__(16) unsigned char afgpu_smem[];
-  const long long na = pa.n_tiles, total = (long long)pa.n_tiles + pb.n_tiles;
-  const int b = blockIdx.x;
-  const int ca = (int)((long long)b * na / total);
-  const bool is_a = (int)(((long long)b + 1) * na / total) > ca;
-  cx<T>* lds = reinterpret_cast<cx<T>*>(afgpu_smem);
-  if (is_a)
-    fft_tile<T, LA, CTA, true, EPI_TW4, false, true, SVA, HXA>(pa, ca, threadIdx.x, lds);
-  else
-    fft_tile<T, LB, CTB, false, EPI_PLAIN, true, true, SVB, HXB>(pb, b - ca, threadIdx.x, lds);
-}
-
 }  // namespace afgpu

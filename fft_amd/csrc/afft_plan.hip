@@ -148,7 +148,7 @@ struct Variant {
   idx_t chunk = 0;    // transforms per pass group (0 = default)
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
   int half_lds = 0;   // 1 / 0 (default): half-LDS kernels where compiled, -1: never
-  int pipeline = 0;   // 1: fused launches of pass A (group k) and pass B (group k-1); 0 (default) / -1: never
+  int pipeline = 0;   // removed schedule (pass A of group k beside pass B of group k-1): must be <= 0
   int real_pair = 0;  // fused real passes: 1 = all (also 1-D r2c pass B), 0 (default) = the N-D row-pair one, -1 = none
   int a_split = 0;    // two-pass plans: launches of pass A per group (0 / 1: one)
   int row_log2 = 0;   // 1-D four-step: row length n2 = 2^row_log2 (0 = default split)
@@ -179,11 +179,6 @@ struct Plan {
   PassRec pass[kMaxPass];
   hipStream_t stream = nullptr;
   int device = 0;
-  // fused schedule of two-pass four-step plans over several groups: launch
-  // k runs pass A of group k and pass B of group k-1 (DualVariant); scratch
-  // 0 then holds two group buffers, used alternately
-  const void* dual = nullptr;
-  int pipe = 0;
   // both passes of a 1-D FP32 c2c four-step in one persistent launch
   // (afft_fused.hip): ring slots per XCD and the pass-B lag, 0 = off
   int fused_slots = 0, fused_lag = 0;
@@ -1209,7 +1204,11 @@ struct Builder {
         if (!q.mfma_mats) ok = false;
       }
     }
-    if (r && ok) pick_dual();
+    if (r && ok && P.var.pipeline > 0) {
+      set_err("variant pipeline is no longer provided (pass A of one group beside pass B of the previous one "
+              "measured slower on every shape); use pipeline = 0");
+      ok = false;
+    }
     if (r && ok) pick_fused();
     return r && ok;
   }
@@ -1225,7 +1224,7 @@ struct Builder {
       ok = false;
       return;
     }
-    if (P.var.fused <= 0 || sizeof(T) != 4 || P.npass != 2 || P.dual || P.mode != afft_gpu_mode_fma ||
+    if (P.var.fused <= 0 || sizeof(T) != 4 || P.npass != 2 || P.mode != afft_gpu_mode_fma ||
         P.kind > 1 || P.ndim != 1 || P.format != afft_interleaved)
       return;
     const PassRec& a = P.pass[0];
@@ -1239,32 +1238,6 @@ struct Builder {
     P.fused_slots = P.var.fused_slots > 0 ? P.var.fused_slots : 3;
     P.fused_lag = P.var.fused_lag > 0 ? P.var.fused_lag : 1;
     if (P.fused_lag >= P.fused_slots) P.fused_lag = P.fused_slots - 1;
-  }
-
-  // Two-pass four-step plans whose pass shapes have a fused kernel run as
-  // launches of (pass A of group k, pass B of group k-1); pass A then uses the
-  // fused kernel's tile shape also when it runs on its own.
-  void pick_dual() {
-    P.dual = nullptr;
-    if (P.var.pipeline <= 0 || P.npass != 2 || P.mode != afft_gpu_mode_fma) return;
-    PassRec& a = P.pass[0];
-    PassRec& b = P.pass[1];
-    if (a.type != PT_FFT || b.type != PT_FFT || !a.colmap || a.epi != EPI_TW4 || a.store_t || b.colmap ||
-        !b.store_t || b.epi != EPI_PLAIN || a.in.role != R_SRC || a.out.role != R_SCR || b.in.role != R_SCR ||
-        b.out.role != R_DST || !a.in.il || !a.out.il || !b.in.il || !b.out.il || a.mfma || b.mfma)
-      return;
-    const DualVariant<T>* d = find_dual<T>(a.L, b.L, b.CT, b.sv, b.hx);
-    if (!d) return;
-    const PassVariant<T>* va = find_pass<T>(a.L, d->cta, 1, EPI_TW4, 0, d->sva, d->hxa);
-    if (!va) return;
-    const void* tw = get_table<T>(0, a.L, va->sv);
-    if (!tw) return;
-    a.variant = va;
-    a.CT = va->CT;
-    a.sv = va->sv;
-    a.hx = va->hx;
-    a.stage_tw = tw;
-    P.dual = d;
   }
 
   bool build_passes(int kind, int format, const idx_t* d, int nd) {
@@ -1911,7 +1884,7 @@ bool split_shape(const Plan& P) {
 // 0.402 ms and keeps its groups of 8.
 constexpr int kAutoSplit = 4;
 bool auto_split(const Plan& P) {
-  if (P.var.a_split != 0 || P.var.chunk != 0 || P.var.pipeline || !split_shape(P) || !P.pass[0].colmap ||
+  if (P.var.a_split != 0 || P.var.chunk != 0 || !split_shape(P) || !P.pass[0].colmap ||
       P.pass[0].L < 1024)
     return false;
   if (P.npass == 2) return true;
@@ -1920,7 +1893,7 @@ bool auto_split(const Plan& P) {
 }
 
 int a_split(const Plan& P) {
-  if (P.pipe || !split_shape(P)) return 1;
+  if (!split_shape(P)) return 1;
   if (P.var.a_split > 1) return P.var.a_split;
   if (!auto_split(P)) return 1;
   // pass A launches of one 64 MiB group each
@@ -1946,11 +1919,11 @@ template <typename T>
 bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hipStream_t st, int first, int last,
                 void* scr, idx_t chunk = 0, bool plain = false) {
   // chunk > 0: groups of that many transforms (scratch sized for it) and,
-  // with plain, one launch per pass only (no fused or pipelined schedule)
+  // with plain, one launch per pass only (no fused schedule)
   if (last < 0) last = P.npass;
   if (chunk <= 0) chunk = P.chunk;
   // scratch 1 follows scratch 0 of the group (the pool layout also makes
-  // room for the fused ring or the pipelined halves, which plain runs skip)
+  // room for the fused ring, which plain runs skip)
   const size_t scr1_off = !plain ? scratch1_offset(P)
                                  : ((size_t)P.scr_per_user[0] * (size_t)chunk * sizeof(cx<T>) + 255) & ~(size_t)255;
   Bufs<T> B{(const T*)sre, (const T*)sim, (T*)dre, (T*)dim,
@@ -1983,41 +1956,6 @@ bool execute_on(Plan& P, const void* sre, const void* sim, void* dre, void* dim,
     f.ctr = (int*)((char*)B.scr[0] + fused_ring_bytes(P));
     f.spin_limit = 1LL << 26;
     HIP_OK(fused_launch(f, a.L, a.CT, a.sv, a.hx, b.L, b.CT, b.sv, b.hx, st));
-    return true;
-  }
-  if (P.pipe && !plain && first == 0 && last == P.npass) {
-    // launch k: pass A of group k into scratch half k%2 beside pass B of
-    // group k-1 from half (k-1)%2; the launches are ordered on `st`
-    const DualVariant<T>* d = (const DualVariant<T>*)P.dual;
-    const size_t half = P.scr_bytes[0] / 2;
-    const idx_t ngroups = (P.batch + P.chunk - 1) / P.chunk;
-    PassParams<T> pb{};
-    bool fast_b = true;
-    for (idx_t k = 0; k <= ngroups; k++) {
-      PassParams<T> pa{};
-      bool fast_a = true;
-      if (k < ngroups) {
-        const idx_t start = k * P.chunk;
-        const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
-        Bufs<T> Bk = B;
-        Bk.scr[0] = (T*)((char*)B.scr[0] + (k & 1) * half);
-        if (!fft_params<T>(P.pass[0], Bk, start, cnt, pa, fast_a)) return false;
-      }
-      if (!fast_a || !fast_b) {
-        set_err("internal: fused four-step launch needs the buffer path on both passes");
-        return false;
-      }
-      HIP_OK(d->launch(pa, pb, st));
-      // pass B of this group runs in the next launch
-      pb = PassParams<T>{};
-      if (k < ngroups) {
-        const idx_t start = k * P.chunk;
-        const idx_t cnt = P.batch - start < P.chunk ? P.batch - start : P.chunk;
-        Bufs<T> Bk = B;
-        Bk.scr[0] = (T*)((char*)B.scr[0] + (k & 1) * half);
-        if (!fft_params<T>(P.pass[1], Bk, start, cnt, pb, fast_b)) return false;
-      }
-    }
     return true;
   }
   const int split = a_split(P);
@@ -2130,10 +2068,7 @@ bool execute(Plan& P, const void* sre, const void* sim, void* dre, void* dim, hi
 
 // Scratch bytes one execute needs (taken from the pool when it runs).
 void size_scratch(Plan& P, size_t elem_bytes) {
-  const int pipe = (P.dual && P.batch > P.chunk) ? 1 : 0;
-  P.pipe = pipe;
-  for (int i = 0; i < 2; i++)
-    P.scr_bytes[i] = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes * (i == 0 && pipe ? 2 : 1);
+  for (int i = 0; i < 2; i++) P.scr_bytes[i] = (size_t)P.scr_per_user[i] * (size_t)P.chunk * elem_bytes;
   if (P.fused_slots)  // the ring and its counters in scratch 0 (pass ranges still run unfused)
     P.scr_bytes[0] = std::max(P.scr_bytes[0], fused_ring_bytes(P) + sizeof(int) * fused_counter_ints(P.batch));
 }
@@ -2382,20 +2317,18 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
   // (streaming) or won (XCD order) on a measured shape
   for (int col : {0, 8, 32})
     for (int rowt : {0, 4, 16})
-      for (int pl : {0, 1})
-        for (int hl : {0, -1})
-          for (int sch : {0, -1})
-            for (int cm : {1, 2, 4}) {
-              Variant v;
-              v.sched = sch;
-              v.col_tile = col;
-              v.rowt_tile = rowt;
-              v.pipeline = pl;
-              v.half_lds = hl;
-              v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
-              if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
-              cands.push_back(v);
-            }
+      for (int hl : {0, -1})
+        for (int sch : {0, -1})
+          for (int cm : {1, 2, 4}) {
+            Variant v;
+            v.sched = sch;
+            v.col_tile = col;
+            v.rowt_tile = rowt;
+            v.half_lds = hl;
+            v.chunk = cm == 1 ? 0 : (cm == 2 ? base_chunk / 2 : base_chunk * 2);
+            if (cm != 1 && (v.chunk < 1 || v.chunk > P.batch || base_chunk >= P.batch)) continue;
+            cands.push_back(v);
+          }
   // plans with a real pass that has a fused form also try the other form
   const bool long_1d_real =
       sizeof(T) == 4 && P.kind >= 2 && P.ndim == 1 && P.dims[0] / 2 > (idx_t)max_row_len<T>();
@@ -2517,7 +2450,7 @@ int describe_pass(const Plan& P, int i, char* buf, size_t len) {
     snprintf(buf, len,
              "fft%s L=%d CT=%d%s%s %s epi=%s%s ncol=%lld bpu=%lld blo=%d in(role=%d il=%d) out(role=%d il=%d)",
              r.mfma ? (r.mfma == afft_gpu_mode_mfma_bf16 ? "[mfma-bf16]" : "[mfma]")
-                    : (P.fused_slots ? "[fused4]" : (P.pipe && i < 2 ? "[fused]" : "")),
+                    : (P.fused_slots ? "[fused4]" : ""),
              r.L, r.CT, r.sv ? " sv1" : "",
              r.hx ? " hx" : "", r.colmap ? "col" : "row", ep[r.epi],
              r.store_t ? " store_t" : "", (long long)r.n_col,

@@ -20,25 +20,6 @@ struct PassVariant {
   PassLaunchFn<T> launch;
 };
 
-// A fused launch of four-step pass A (column pass of length la with the
-// twiddle epilogue) and pass B (transposed-store row pass of length lb), each
-// on its own tile grid; both sides use the buffer-resource path.
-template <typename T>
-using DualLaunchFn = hipError_t (*)(const PassParams<T>& pa, const PassParams<T>& pb, hipStream_t);
-
-template <typename T>
-struct DualVariant {
-  int la, cta, sva, hxa;
-  int lb, ctb, svb, hxb;
-  int threads;
-  DualLaunchFn<T> launch;
-};
-
-// The fused kernel for pass A of length la and pass B of length lb with B's
-// tile (ctb rows, schedule svb, half exchange hxb), or nullptr.
-template <typename T>
-const DualVariant<T>* find_dual(int la, int lb, int ctb, int svb, int hxb);
-
 // Looks up the compiled variant; returns nullptr when none exists.
 template <typename T>
 const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t, int sv = 0, int hx = 0);

@@ -124,9 +124,9 @@ typedef struct
                         where compiled, -1: never */
   int32_t half_lds;  /* 1: half-LDS kernels (real and imaginary parts exchanged in turn)
                         where compiled, -1: never, 0: default (on) */
-  int32_t pipeline;  /* two-pass four-step plans over several groups: 1 runs pass A of
-                        group k and pass B of group k-1 in one fused launch (double-
-                        buffered scratch); 0 (default) or -1: one pass per launch */
+  int32_t pipeline;  /* must be 0 or -1: the schedule that ran pass A of group k beside
+                        pass B of group k-1 in one launch measured slower on every shape
+                        and is no longer provided (1 is refused with a message) */
   int32_t real_pair; /* fused real passes.  N-D r2c / c2r with dim0 > 4096: the
                         (inverse) real pass over row pairs (k, h-k) runs inside
                         pass B (r2c) or pass A (c2r) of the dim-0 column

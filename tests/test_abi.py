@@ -365,6 +365,13 @@ def test_plan_model_three_pass_fourstep():
     assert [(p.type, p.L) for p in passes] == [(0, 256), (0, 256), (0, 2048), (1, 1)]
 
 
+def test_removed_pipeline_variant_is_refused():
+    with pytest.raises(ValueError, match="pipeline"):
+        M.inspect("c", (1 << 20,), IL, 8, variant={"pipeline": 1})
+    info, passes = M.inspect("c", (1 << 20,), IL, 8, variant={"pipeline": -1})
+    assert len(passes) == 2
+
+
 def test_unsupported_shapes_report_errors():
     with pytest.raises(ValueError):
         M.inspect("c", (1 << 26, 2), IL, 1)

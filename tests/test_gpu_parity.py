@@ -326,8 +326,8 @@ def test_errors_leave_state_clean(G):
     assert "power of two" in _lib.last_error()
 
 
-VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, pipeline=pl, chunk=ch, real_pair=rp)
-            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8, 32) for pl, ch in ((1, 1), (0, 0))
+VARIANTS = [dict(sched=s, half_lds=h, rowt_tile=r, col_tile=c, chunk=ch, real_pair=rp)
+            for s in (0, -1) for h in (0, -1) for r in (0, 16) for c in (0, 8, 32) for ch in (1, 0)
             for rp in (1, -1)]
 
 
@@ -457,24 +457,25 @@ def test_split_pass_a_matches_one_launch(G, kind, n, batch, chunk, split):
     p.close()
 
 
-def test_pipelined_plan_graph_replay(G):
-    """A four-step plan over several groups runs pass B of one group on the
-    plan's second stream while pass A of the next runs on the caller's; the
-    result must equal the serial schedule bit for bit, eagerly and when the
-    whole execute is captured in a HIP graph and replayed (with eager work and
-    synchronisations between replays)."""
+def test_grouped_plan_graph_replay(G):
+    """A four-step plan over several groups (one transform each), captured
+    whole in a HIP graph and replayed with eager work and synchronisations
+    between the replays, equals the eager execute bit for bit; the removed
+    pipelined schedule is refused with a message."""
     import torch
     from fft_amd.afft import Plan
     n, batch = 1 << 20, 6
-    say("pipelined plan + graph", n, batch)
+    say("grouped plan + graph", n, batch)
     x = O.complex_input(41, (batch, n))
     src = G.to_dev(np.stack([x.real, x.imag], -1).astype(np.float32).ravel())
     ref = G.empty_dev(2 * n * batch)
     p = Plan("c", (n,), batch)
-    p.set_variant(chunk=1, pipeline=0)
+    p.set_variant(chunk=1)
     p(src, None, ref, None)
     torch.cuda.synchronize()
-    p.set_variant(chunk=1, pipeline=1)
+    with pytest.raises(RuntimeError, match="pipeline"):
+        p.set_variant(chunk=1, pipeline=1)
+    p.set_variant(chunk=1)
     dst = G.empty_dev(2 * n * batch)
     p(src, None, dst, None)
     torch.cuda.synchronize()
