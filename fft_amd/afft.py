@@ -187,7 +187,8 @@ class Variant(ctypes.Structure):
                 ("xcd_order", ctypes.c_int32), ("chunk", ctypes.c_int64), ("sched", ctypes.c_int32),
                 ("half_lds", ctypes.c_int32), ("pipeline", ctypes.c_int32), ("real_pair", ctypes.c_int32),
                 ("a_split", ctypes.c_int32), ("row_log2", ctypes.c_int32), ("fused", ctypes.c_int32),
-                ("fused_slots", ctypes.c_int32), ("fused_lag", ctypes.c_int32)]
+                ("fused_slots", ctypes.c_int32), ("fused_lag", ctypes.c_int32),
+                ("tiny_rows", ctypes.c_int32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_obj")
 LIB = os.path.join(PKG, "libafft_mi355x.so")
 SOURCES = ["afft_kernels_f32.hip", "afft_kernels_f64.hip", "afft_aux.hip", "afft_plan.hip", "afft_queue.hip", "afft_mfma.hip",
-           "afft_realpair.hip", "afft_fused.hip", "afft_real4.hip"]
+           "afft_realpair.hip", "afft_fused.hip", "afft_real4.hip", "afft_tiny.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

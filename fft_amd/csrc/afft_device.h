@@ -204,6 +204,7 @@ AFGPU_SCHED(2048, 0, 16, 3, 16, 16, 8)
 AFGPU_SCHED(4096, 0, 16, 3, 16, 16, 16)
 AFGPU_SCHED(8192, 0, 16, 4, 16, 16, 16, 2)
 AFGPU_SCHED(16384, 0, 16, 4, 16, 16, 16, 4)
+AFGPU_SCHED(32768, 0, 32, 3, 32, 32, 32)
 AFGPU_SCHED(512, 1, 32, 2, 32, 16)
 AFGPU_SCHED(1024, 1, 32, 2, 32, 32)
 AFGPU_SCHED(2048, 1, 32, 3, 32, 32, 2)

@@ -3,6 +3,7 @@
 // 160 KiB LDS of a CU bounds CT * L.
 #define AFGPU_T float
 #define AFGPU_ROW_MAX 16384
+#define AFGPU_ROW_HX_MAX 32768
 #define AFGPU_ROWT_CT64 16
 #define AFGPU_ROWT_CT128 16
 #define AFGPU_ROWT_CT256 16

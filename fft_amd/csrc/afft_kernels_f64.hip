@@ -1,7 +1,9 @@
 // FP64 instantiations of the pass kernels (16-byte elements halve what fits
-// in LDS, so tiles are half the FP32 ones and rows stop at 8192).
+// in LDS, so tiles are half the FP32 ones and rows stop at 8192, 16384 with
+// half-LDS tiles).
 #define AFGPU_T double
 #define AFGPU_ROW_MAX 8192
+#define AFGPU_ROW_HX_MAX 16384
 #define AFGPU_ROWT_CT64 8
 #define AFGPU_ROWT_CT128 8
 #define AFGPU_ROWT_CT256 8

@@ -344,7 +344,6 @@ struct RtPol {
 
 template <typename T, int L, int CT, bool COLMAP, int EPI, bool STORE_T, bool FAST = false, int SV = 0, bool HX = false>
 __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int tid, cx<T>* __restrict__ lds) {
-  static_assert(!(HX && EPI == EPI_R2C), "the real-output epilogue exchanges whole complex entries");
   static_assert(EPI != EPI_C2R4 || (COLMAP && !STORE_T), "the paired load is for four-step column passes");
   using PS = PassShape<L, CT, SV, (!COLMAP && STORE_T) ? (HX ? (int)sizeof(T) : 2 * (int)sizeof(T)) : 0>;
   constexpr int E = PS::E;
@@ -533,10 +532,30 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   };
   with_out_pol([&](auto pol) AFGPU_INLINE {
     if constexpr (EPI == EPI_R2C) {
-      static_for<E>([&](auto mc) AFGPU_INLINE {
-        constexpr int m = decltype(mc)::value;
-        lat(t + m * TPT) = v[m];
-      });
+      // the partner Z[L-k] of each bin crosses the LDS as a whole entry, or
+      // (HX) as its real part, then its imaginary part
+      T pre[HX ? E : 1];
+      if constexpr (HX) {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          latf(t + m * TPT) = v[m].x;
+        });
+        __syncthreads();
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          pre[m] = latf((L - (t + m * TPT)) & (L - 1));
+        });
+        __syncthreads();
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          latf(t + m * TPT) = v[m].y;
+        });
+      } else {
+        static_for<E>([&](auto mc) AFGPU_INLINE {
+          constexpr int m = decltype(mc)::value;
+          lat(t + m * TPT) = v[m];
+        });
+      }
       __syncthreads();
       if (active) {
         static_for<E>([&](auto mc) AFGPU_INLINE {
@@ -548,7 +567,11 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
             x = {s0.x + s0.y, T(0)};
             store(pol, L, cx<T>{s0.x - s0.y, T(0)});
           } else {
-            cx<T> s1c = conjg(lat(L - k));
+            cx<T> s1c;
+            if constexpr (HX)
+              s1c = {pre[m], -latf(L - k)};
+            else
+              s1c = conjg(lat(L - k));
             cx<T> w = p.real_tw[k];
             const T h = T(0.5);
             cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};

@@ -41,6 +41,7 @@
 #include "afft_mfma.h"
 #include "afft_real4.h"
 #include "afft_registry.h"
+#include "afft_tiny.h"
 
 using namespace afgpu;
 
@@ -84,7 +85,8 @@ enum PassType {
   PT_REAL4A = 9,
   PT_REAL4B = 10,
   PT_REAL4C = 11,
-  PT_REAL4D = 12
+  PT_REAL4D = 12,
+  PT_TINY = 13
 };
 
 struct Side {
@@ -155,6 +157,8 @@ struct Variant {
   int fused = 0;      // 1: both four-step passes in one persistent launch where supported
   int fused_slots = 0;  // ring slots per XCD of the fused launch (0 = 3)
   int fused_lag = 0;    // rounds between pass A and pass B of a transform (0 = 1)
+  int tiny_rows = 0;    // rows of 2..32 points: LDS-staged kernel with 64 / 128 / 256 rows per group
+                        // (0 = default rows), -1: the general row kernel
 };
 
 // A plan lives in the caller's `mem`, as afft's does (fft_internal.hpp:24-106):
@@ -385,9 +389,18 @@ struct Builder {
   // Row length n2 of a 1-D four-step n = n1 * n2: 1024-point rows up to 2^20;
   // beyond, 2048-point rows keep the column pass at <= 2048 points (2^21 as
   // 1024 x 2048 measured 3-5 % faster than 2048 x 1024); a variant may set it.
+  // Longest row pass: the half-LDS-only lengths unless the variant turns the
+  // half-LDS kernels off.
+  idx_t row_max() const { return P.var.half_lds < 0 ? max_row_len_full<T>() : max_row_len<T>(); }
+
+  // Row length n2 of the 1-D four-step n = n1 * n2: 1024-point rows up to
+  // 2^20, but columns of at least 64 points (a pass A over 16- or 32-point
+  // columns measured 1.29 TB/s for FP64 2^14, 2.84 as 64 x 256; FP64 2^15
+  // 2.39 -> 2.75 as 64 x 512); longer rows beyond 2^20.
   idx_t row_len_1d(idx_t n) const {
     if (P.var.row_log2 > 0) return P.var.row_log2 < 40 ? (idx_t)1 << P.var.row_log2 : 0;
-    return n <= ((idx_t)1 << 20) ? 1024 : std::max<idx_t>(2048, n / max_col_len<T>());
+    if (n <= ((idx_t)1 << 20)) return std::min<idx_t>(1024, std::max<idx_t>(2, n / 64));
+    return std::max<idx_t>(2048, n / max_col_len<T>());
   }
 
   // pref_ct > 0: a tile tried before the defaults (split-addressed column
@@ -454,6 +467,21 @@ struct Builder {
     return f(a) && f(b);
   }
 
+  // A row pass of 2..32-point rows whose rows form one stack (every user
+  // transform folded into the row index, entries of a row adjacent) runs on
+  // the LDS-staged one-row-per-thread kernel: its loads and stores are runs
+  // over the work-group's rows instead of one 8-byte entry per row per lane.
+  void tiny(PassRec* r) {
+    if (!ok || P.var.tiny_rows < 0 || r->type != PT_FFT || r->colmap || r->store_t || !r->fold_user) return;
+    const int rows = P.var.tiny_rows > 0 ? P.var.tiny_rows : tiny_default_rows((int)sizeof(T), r->L, r->epi);
+    if (r->in.s_j != 1 || r->out.s_j != 1 || !tiny_supported(r->L, r->epi, rows)) return;
+    r->type = PT_TINY;
+    r->variant = nullptr;
+    r->CT = rows;
+    r->sv = r->hx = 0;
+    r->stage_tw = nullptr;
+  }
+
   void scratch_side(Side& s, int role) {
     s = Side();
     s.role = role;
@@ -471,7 +499,7 @@ struct Builder {
     if (out.role < R_SCR && out.s_u == 0) out.s_u = tens;
     const int tmp_role = (in.role == R_SCR || out.role == R_SCR) ? R_SCR2 : R_SCR;
 
-    if (inner == 1 && n <= max_row_len<T>()) {
+    if (inner == 1 && n <= row_max()) {
       PassRec* r = add();
       if (!r || !fft_pass(r, (int)n, 0, EPI_PLAIN, 0)) return;
       r->in = in;
@@ -490,6 +518,7 @@ struct Builder {
       }
       r->swap_in = swi;
       r->swap_out = swo;
+      tiny(r);
       return;
     }
     // batch stride of a side whose batch index runs over (user, outer)
@@ -1195,6 +1224,16 @@ struct Builder {
       // 64 x 2^20 with the XCD-contiguous tile order)
       if (q.type == PT_REAL4B) q.out_nt = 0;
       if (q.type == PT_REAL4C) q.in_nt = 0;
+      // The same holds for row passes whose waves cover only part of a line of
+      // each row per access (c2c rows of up to 128 points: a wave's access
+      // spans 64 B of each of eight rows) and for real row passes, whose rows
+      // of L+1 bins start off line boundaries: cached, the partial lines of
+      // neighbouring accesses merge in L2 before they reach HBM (256 MiB per
+      // batch: r2c 2^7 1.62 -> 4.21 TB/s, c2r 2^7..2^13 2.6-3.3 -> 4.1-4.5,
+      // c2c 2^6 4.48 -> 5.03; c2c rows of 256 points and more keep streaming,
+      // 2-7 % faster)
+      if (q.type == PT_FFT && !q.colmap && !q.store_t && (q.L <= 128 || q.epi == EPI_R2C || q.epi == EPI_C2R))
+        q.in_nt = q.out_nt = 0;
       q.xcd = P.var.xcd_order;
       q.mfma = 0;
       if (P.mode != afft_gpu_mode_fma && sizeof(T) == 4 && q.type == PT_FFT && q.L == 256 &&
@@ -1203,6 +1242,11 @@ struct Builder {
         q.mfma_mats = get_table<T>(3, 256, P.mode == afft_gpu_mode_mfma_bf16 ? 1 : 0);
         if (!q.mfma_mats) ok = false;
       }
+    }
+    if (r && ok && P.var.tiny_rows > 0 && P.var.tiny_rows != 64 && P.var.tiny_rows != 128 &&
+        P.var.tiny_rows != 256) {
+      set_err("variant tiny_rows must be 64, 128 or 256 (rows per work-group), 0 (default) or -1");
+      ok = false;
     }
     if (r && ok && P.var.pipeline > 0) {
       set_err("variant pipeline is no longer provided (pass A of one group beside pass B of the previous one "
@@ -1273,7 +1317,11 @@ struct Builder {
         in.s_u = h;
         Side out = usr_dst;
         out.s_u = h + 1;
-        if (h <= max_row_len<T>()) {
+        // beyond the full-LDS rows the real-input four-step comes first (FP32
+        // 2^16 x 512: 0.128 ms against 0.173 for one half-LDS row pass, whose
+        // r2c epilogue spills at 32 elements per thread)
+        if (h > max_row_len_full<T>() && r2c_real4(in, out, h)) return ok;
+        if (h <= row_max()) {
           PassRec* r = add();
           if (!r || !fft_pass(r, (int)h, 0, EPI_R2C, 0)) return false;
           r->in = in;
@@ -1283,6 +1331,7 @@ struct Builder {
           r->out.s_c = h + 1;
           r->n_col = 1;
           r->fold_user = 1;
+          tiny(r);
           return ok;
         }
         if (r2c_real4(in, out, h)) return ok;
@@ -1370,7 +1419,11 @@ struct Builder {
       Side out = usr_dst;
       out.il = 1;  // N reals written as N/2 interleaved complex
       out.s_u = h;
-      if (h <= max_row_len<T>()) {
+      // variant real_pair = 2 asks for the real-input four-step wherever it
+      // has kernels; by default the half-LDS row pass holds 2^16 (FP32 2^16 x
+      // 512: 0.113 ms against 0.155 for the real-input four-step)
+      if (P.var.real_pair == 2 && h > max_row_len_full<T>() && c2r_real4(in, out, h)) return ok;
+      if (h <= row_max()) {
         PassRec* r = add();
         if (!r || !fft_pass(r, (int)h, 0, EPI_C2R, 0)) return false;
         r->in = in;
@@ -1381,6 +1434,7 @@ struct Builder {
         r->n_col = 1;
         r->fold_user = 1;
         r->swap_in = r->swap_out = 1;
+        tiny(r);
         return ok;
       }
       if (c2r_real4(in, out, h)) return ok;
@@ -1767,6 +1821,25 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.stage_tw = (const cx<T>*)r.stage_tw;
     a.real_tw = (const cx<T>*)r.real_tw;
     HIP_OK(real_colpair<T>(a, st));
+    return true;
+  }
+  if (r.type == PT_TINY) {
+    TinyArgs<T> a{};
+    a.in_re = ire;
+    a.in_im = iim;
+    a.in_il = r.in.il;
+    a.in_nt = r.in_nt;
+    a.out_re = ore;
+    a.out_im = oim;
+    a.out_il = r.out.il;
+    a.out_nt = r.out_nt;
+    a.rows = r.n_col * cnt;
+    a.in_rs = r.in.s_c;
+    a.out_rs = r.out.s_c;
+    a.swap_in = r.swap_in;
+    a.swap_out = r.swap_out;
+    a.real_tw = (const cx<T>*)r.real_tw;
+    HIP_OK(tiny_rows<T>(a, r.L, r.epi, r.CT, st));
     return true;
   }
   if (r.type == PT_REAL4C || r.type == PT_REAL4D) {
@@ -2244,7 +2317,7 @@ TuneCache& tcache() {
 
 // The cache file (AFFT_GPU_TUNE_CACHE) holds one line per tuned shape:
 //   <key> <col_tile> <rowt_tile> <stream_io> <xcd_order> <chunk> <sched> [<half_lds> [<pipeline> [<real_pair>
-//   [<a_split> [<row_log2> [<fused> [<fused_slots> [<fused_lag>]]]]]]]]
+//   [<a_split> [<row_log2> [<fused> [<fused_slots> [<fused_lag> [<tiny_rows>]]]]]]]]]
 void tcache_load_locked(TuneCache& c) {
   if (c.loaded) return;
   c.loaded = true;
@@ -2256,9 +2329,10 @@ void tcache_load_locked(TuneCache& c) {
   while (fgets(line, sizeof(line), f)) {
     Variant v;
     long long ch = 0;
-    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d", key, &v.col_tile,
+    const int got = sscanf(line, "%511s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d %d", key, &v.col_tile,
                            &v.rowt_tile, &v.stream_io, &v.xcd_order, &ch, &v.sched, &v.half_lds, &v.pipeline,
-                           &v.real_pair, &v.a_split, &v.row_log2, &v.fused, &v.fused_slots, &v.fused_lag);
+                           &v.real_pair, &v.a_split, &v.row_log2, &v.fused, &v.fused_slots, &v.fused_lag,
+                           &v.tiny_rows);
     if (got < 7) continue;
     v.chunk = ch;
     c.best[key] = v;
@@ -2285,9 +2359,9 @@ void tcache_put(const Plan& P, const Variant& v) {
   if (!path) return;
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d\n", plan_key(P).c_str(), v.col_tile, v.rowt_tile,
-          v.stream_io, v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair, v.a_split,
-          v.row_log2, v.fused, v.fused_slots, v.fused_lag);
+  fprintf(f, "%s %d %d %d %d %lld %d %d %d %d %d %d %d %d %d %d\n", plan_key(P).c_str(), v.col_tile,
+          v.rowt_tile, v.stream_io, v.xcd_order, (long long)v.chunk, v.sched, v.half_lds, v.pipeline, v.real_pair,
+          v.a_split, v.row_log2, v.fused, v.fused_slots, v.fused_lag, v.tiny_rows);
   fclose(f);
 }
 
@@ -2434,6 +2508,7 @@ int stage_list(int L, int sv, int* radix, int* ns) {
     case 4096: return fill_stages<4096, 0>(radix, ns);
     case 8192: return fill_stages<8192, 0>(radix, ns);
     case 16384: return fill_stages<16384, 0>(radix, ns);
+    case 32768: return fill_stages<32768, 0>(radix, ns);
     default: return 0;
   }
 }
@@ -2481,6 +2556,9 @@ int describe_pass(const Plan& P, int i, char* buf, size_t len) {
   else if (r.type == PT_REAL4D)
     snprintf(buf, len, "real4d n1=%d CT=%d sv%d %s n2=%lld in(role=%d) out(role=%d)", r.L, r.CT, r.sv, r.hx ? "hx" : "lds", (long long)r.rr_m,
              r.in.role, r.out.role);
+  else if (r.type == PT_TINY)
+    snprintf(buf, len, "tiny L=%d rows=%d epi=%s ncol=%lld in(role=%d il=%d) out(role=%d il=%d)", r.L, r.CT,
+             ep[r.epi], (long long)r.n_col, r.in.role, r.in.il, r.out.role, r.out.il);
   else if (r.type == PT_REALCOLPAIR)
     snprintf(buf, len, "real_colpair L=%d CT=%d h1=%d h=%lld m=%lld out(role=%d il=%d)", r.L, r.CT, r.n_blo,
              (long long)r.rr_h, (long long)r.rr_m, r.out.role, r.out.il);
@@ -2523,6 +2601,7 @@ Plan* dry_plan(int precision, int kind, size_t ndim, const size_t* dim, int form
     P->var.fused = v->fused;
     P->var.fused_slots = v->fused_slots;
     P->var.fused_lag = v->fused_lag;
+    P->var.tiny_rows = v->tiny_rows;
   }
   g_dry = true;
   bool ok = precision == 32 ? Builder<float>(*P).build(kind, format, d, nd)
@@ -2758,6 +2837,7 @@ int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v) {
   P.var.fused = v->fused;
   P.var.fused_slots = v->fused_slots;
   P.var.fused_lag = v->fused_lag;
+  P.var.tiny_rows = v->tiny_rows;
   if (!rebuild_any(P)) {
     P.var = old;
     rebuild_any(P);
@@ -2813,6 +2893,7 @@ int afft_gpu_plan_get_variant(const afft_gpu_plan* plan, afft_gpu_variant* v) {
   v->fused = P.var.fused;
   v->fused_slots = P.var.fused_slots;
   v->fused_lag = P.var.fused_lag;
+  v->tiny_rows = P.var.tiny_rows;
   return 0;
 }
 

@@ -27,7 +27,9 @@ const PassVariant<T>* find_pass(int L, int CT, int colmap, int epi, int store_t,
 // Largest L with a compiled row / column kernel, and the column tile (CT) used
 // for a column pass of length L.
 template <typename T>
-int max_row_len();
+int max_row_len();  // longest row pass (the longest ones have half-LDS kernels only)
+template <typename T>
+int max_row_len_full();  // longest row pass with a full-LDS kernel
 template <typename T>
 int max_col_len();
 template <typename T>

@@ -138,7 +138,8 @@ typedef struct
                         2 runs the real-input four-step (two passes over N/2
                         complex: pass A reads two real columns as one complex
                         column, pass B stores each bin once; 2^16 <= N <= 2^23),
-                        the default for r2c up to 2^23 and c2r up to 2^22.
+                        the default for r2c up to 2^23 and c2r 2^17..2^22 (c2r
+                        2^16 runs as one half-LDS row pass).
                         Longer ones (or -1) run the complex four-step of the
                         N/2 packed points with the real pass on its own: r2c
                         as a row pass into a second scratch and the transposing
@@ -152,8 +153,9 @@ typedef struct
                         pass B in one (0 or 1: one) */
   int32_t row_log2;  /* 1-D four-step plans: row length n2 = 2^row_log2 of the split
                         n = n1 * n2 (pass A: n2 columns of n1 points, pass B: n1 rows
-                        of n2 points); 0 (default): 1024 up to 2^20, longer rows
-                        beyond */
+                        of n2 points); 0 (default): 1024 up to 2^20 (n/64 where
+                        that is shorter, so columns have at least 64 points),
+                        longer rows beyond */
   int32_t fused;     /* batched 1-D FP32 c2c four-step plans: 1 would run both passes in
                         one persistent launch (pass-A and pass-B tiles of each transform
                         on one XCD, the intermediate in a ring of scratch slots that
@@ -163,6 +165,13 @@ typedef struct
   int32_t fused_slots; /* ring slots per XCD of the fused launch (0: 3) */
   int32_t fused_lag;   /* rounds between a transform's pass A and its pass B in the
                           fused launch (0: 1; < fused_slots) */
+  int32_t tiny_rows;   /* rows of 2..32 complex points (c2c N <= 32, 1-D r2c / c2r
+                          N <= 64, last axes of N-D c2c of that length) can run one
+                          row per thread, each work-group's rows staged through LDS
+                          so its loads and stores are contiguous: 64, 128 or 256
+                          rows per work-group; 0 (default): so for rows of up to 16
+                          points (by row size), 32-point rows on the general row
+                          kernel; -1: the general row kernel for all */
 } afft_gpu_variant;
 
 int afft_gpu_plan_set_variant(afft_gpu_plan* plan, const afft_gpu_variant* v);
@@ -290,7 +299,15 @@ typedef struct
                    8 = N-D c2r: the inverse real pass over row pairs fused
                        into pass A of the dim-0 column four-step h = L * h2
                        (input rows 0..h at in.s_j, output entry (k1, n2, c) at
-                       k1*out.s_j + n2*rr_m + c; n_blo = h2, rr_h = h) */
+                       k1*out.s_j + n2*rr_m + c; n_blo = h2, rr_h = h),
+                   9 / 10 = 1-D r2c real-input four-step pass A (L = n1-point
+                       columns, rr_m = n2) / pass B (L = n2-point rows,
+                       rr_m = n1), rr_h = N/2,
+                   11 / 12 = 1-D c2r real-input four-step pass B' (L = n2,
+                       rr_m = n1) / pass A' (L = n1, rr_m = n2),
+                   13 = rows of L = 2..32 points, one per thread, CT rows per
+                       work-group staged through LDS (row r at r*s_c, entries
+                       adjacent; epi plain, r2c or c2r as for type 0) */
   int32_t L, CT, colmap, epi, store_t;
   afft_gpu_side_info in, out;
   int64_t n_col;

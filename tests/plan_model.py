@@ -578,6 +578,21 @@ def _real4d(p, bufs, start, cnt, pi):
     _scatter(p.out, bufs, e_out, z, f"pass {pi} store")
 
 
+def _tiny(p, bufs, start, cnt, pi):
+    """Rows of 2..32 points, one per thread, staged through LDS (tiny_rows_kernel,
+    fft_amd/csrc/afft_tiny.hip): the kernel addresses row g of the launch
+    (g < n_col * cnt, every user transform folded into the row index) at
+    g * s_c and its entries at +j, so the pass must be a folded row pass with
+    adjacent entries; its arithmetic is the row pass's (plain, r2c post-pass
+    or c2r pre-pass, re / im exchanged on each side as asked)."""
+    if not (p.fold_user and p.in_.s_j == 1 and p.out.s_j == 1 and p.epi in (0, 2, 3) and p.L in (2, 4, 8, 16, 32)
+            and not p.colmap and not p.store_t and p.n_blo == 1):
+        raise AssertionError(f"pass {pi}: tiny pass outside the kernel's contract")
+    if p.epi == 3 and not (p.swap_in and p.swap_out):
+        raise AssertionError(f"pass {pi}: the tiny c2r kernel always exchanges re / im on both sides")
+    _fft_pass(p, bufs, start, cnt, pi)
+
+
 def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant=None):
     """Executes the plan model; returns the dst arrays (float64).  edit
     (optional) maps the inspected pass list to the one executed; variant
@@ -618,6 +633,8 @@ def run(kind, dims, fmt, batch, src_re, src_im=None, prec=32, edit=None, variant
                 _real4c(p, bufs, start, cnt, pi)
             elif p.type == 12:
                 _real4d(p, bufs, start, cnt, pi)
+            elif p.type == 13:
+                _tiny(p, bufs, start, cnt, pi)
             else:
                 for u in range(start, start + cnt):
                     if not p.rr_inv:

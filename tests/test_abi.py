@@ -147,6 +147,45 @@ def test_plan_model_matches_oracle(kind, dims):
             assert err < 1e-9, (kind, dims, fmt, batch, err)
 
 
+@pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])
+def test_plan_model_tiny_rows(kind):
+    # rows of 2..16 points (c2c N <= 16, r2c / c2r N <= 32, last axes of N-D
+    # shapes) run on the LDS-staged one-row-per-thread kernel (type 13), and
+    # 32-point rows on request (variant tiny_rows > 0); variant tiny_rows = -1
+    # keeps the general row kernel (type 0)
+    real = kind in ("r", "ri")
+    for lg in range(1, 7 if real else 6):
+        n = 1 << lg
+        for fmt, batch, prec in ((IL, 1, 32), (SP, 300, 32), (IL, 5, 64)):
+            err, info, passes = model_check(kind, (n,), fmt, batch, prec)
+            L = n // 2 if real else n
+            # 32-point rows default to the general kernel (faster there)
+            want = 13 if 2 <= L <= 16 else 0
+            assert [p.type for p in passes][-1:] == [want], (n, [(p.type, p.L) for p in passes])
+            assert err < 1e-9, (kind, n, fmt, batch, err)
+            err, info, passes = model_check(kind, (n,), fmt, batch, prec, variant={"tiny_rows": -1})
+            assert 13 not in [p.type for p in passes] and err < 1e-9
+            if L >= 2:
+                # an explicit work-group size asks for the tiny kernel
+                err, info, passes = model_check(kind, (n,), fmt, batch, prec, variant={"tiny_rows": 64})
+                assert passes[-1].type == 13 and passes[-1].CT == 64 and err < 1e-9
+    for bad in (1, 32, 100, 512):
+        with pytest.raises(ValueError, match="tiny_rows"):
+            M.inspect(kind, (16,), IL, 4, variant={"tiny_rows": bad})
+    # N-D: the last axis's rows (c2c), later axes on the general kernels
+    for dims in ((8, 8, 8), (64, 16), (4, 16)) if not real else ((8, 8, 8),):
+        err, info, passes = model_check(kind, dims, SP, 2)
+        assert err < 1e-9, (dims, err)
+        if not real:
+            assert passes[0].type == 13 and passes[0].L == dims[-1], [(p.type, p.L) for p in passes]
+    # 64-point c2c rows and the one-point real transform stay on the general kernel
+    info, passes = M.inspect("c", (64,), IL, 8)
+    assert [p.type for p in passes] == [0]
+    if real:
+        info, passes = M.inspect(kind, (2,), IL, 8)
+        assert 13 not in [p.type for p in passes]
+
+
 @pytest.mark.parametrize("kind", ["r", "ri"])
 def test_plan_model_real_pass_in_rows(kind):
     # long real dim 0 with a 1024-point last axis: by default the real pass
@@ -204,11 +243,12 @@ def test_plan_model_c2r_pre_pass_in_pass_a(lg):
             assert err < 1e-9, (lg, fmt, v, err)
 
 
-@pytest.mark.parametrize("lg", [16, 17, 20, 22])
+@pytest.mark.parametrize("lg", [17, 20, 22])
 def test_plan_model_fused_real_pass(lg):
     # the opt-in fused real pass (rows k1 and n1 - k1 per work-group, 1024- or
     # 2048-point rows): slot mapping, partner bins, row 0 / row n1/2 and the
-    # Nyquist bin against the oracle, both formats, a batch of 3
+    # Nyquist bin against the oracle, both formats, a batch of 3 (2^16 splits
+    # its 2^15 packed points as 64 x 512, which has no fused kernel)
     for fmt in (IL, SP):
         err, info, passes = model_check("r", (1 << lg,), fmt, 3 if lg < 22 else 1, variant={"real_pair": 1})
         assert [p.type for p in passes][-1] == 4, [p.type for p in passes]
@@ -240,9 +280,11 @@ def test_real_input_fourstep_shape_variants():
     assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(9, 2048, 16, 1), (10, 2048, 8, 0)]
     info, p = M.inspect("ri", (1 << 22,), IL, 2, variant={"real_pair": 2})
     assert [(q.type, q.L, q.CT, q.sv) for q in p] == [(11, 2048, 8, 1), (12, 2048, 8, 0)]
-    # defaults by length (measured): r2c 2^16..2^23, c2r 2^16..2^22
-    for kind, lg, real4 in (("r", 16, True), ("r", 22, True), ("r", 23, True), ("r", 24, False), ("ri", 16, True),
-                            ("ri", 20, True), ("ri", 22, True), ("ri", 23, False)):
+    # defaults by length (measured): r2c 2^16..2^23, c2r 2^17..2^22 (c2r 2^16 is
+    # one half-LDS row pass)
+    for kind, lg, real4 in (("r", 16, True), ("r", 17, True), ("r", 22, True), ("r", 23, True), ("r", 24, False),
+                            ("ri", 16, False), ("ri", 17, True), ("ri", 20, True), ("ri", 22, True),
+                            ("ri", 23, False)):
         info, p = M.inspect(kind, (1 << lg,), IL, 1)
         assert (p[0].type in (9, 11)) == real4, (kind, lg, [q.type for q in p])
     info, p = M.inspect("r", (1 << 22,), IL, 2, variant={"real_pair": 2, "rowt_tile": 16, "sched": 1, "col_tile": 16})
@@ -339,7 +381,7 @@ def test_plan_shapes_of_headline_configs():
     info, passes = M.inspect("r", (8192, 64), IL, 1)
     assert [p.type for p in passes] == [0, 7, 0] and passes[1].rr_h == 4096 and passes[1].rr_m == 64
     info, passes = M.inspect("r", (8192, 8), IL, 1)
-    assert [p.type for p in passes] == [0, 0, 1, 0] and passes[2].rr_h == 4096 and passes[2].rr_m == 8
+    assert [p.type for p in passes] == [0, 0, 1, 13] and passes[2].rr_h == 4096 and passes[2].rr_m == 8
     # 2048 x 8192 r2c: dim 0 short enough for one fused column r2c pass
     info, passes = M.inspect("r", (2048, 8192), IL, 1)
     assert [(p.L, p.colmap, p.epi) for p in passes] == [(1024, 1, 2), (8192, 0, 0)]

@@ -16,13 +16,13 @@ import plan_model as M
 
 NAMES = {0: "fft", 1: "real_rows", 2: "real_one", 3: "real_tr", 4: "real_pair", 5: "real_rowfft",
          6: "real_irowfft", 7: "real_colpair", 8: "real_icolpair", 9: "real4a", 10: "real4b", 11: "real4c",
-         12: "real4d"}
+         12: "real4d", 13: "tiny"}
 
 SHAPES = [(1,), (2,), (64,), (4096,), (16384,), (1 << 15,), (1 << 16,), (1 << 20,), (1 << 21,), (1 << 22,),
           (1 << 24,), (1 << 25,), (16, 32), (8192, 64), (8192, 1024), (16384, 2048), (256, 256, 256), (8, 4, 16, 2),
           (4096, 16), (65536, 16)]
 VARIANTS = [None, dict(real_pair=1), dict(real_pair=-1), dict(real_pair=2), dict(fused=1), dict(col_tile=32),
-            dict(sched=-1, half_lds=-1), dict(rowt_tile=16), dict(a_split=2), dict(row_log2=9)]
+            dict(sched=-1, half_lds=-1), dict(rowt_tile=16), dict(a_split=2), dict(row_log2=9), dict(tiny_rows=-1)]
 
 
 def _describe(prec, kind, shape, fmt, batch, v, i, size=512):

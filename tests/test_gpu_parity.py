@@ -194,8 +194,8 @@ def test_nd_c2r_real_pass_forms(G, shape):
     assert any("real_rows" in d for s_ in seen for d in s_), seen
 
 
-@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64),
-                                   (16384, 64), (65536, 16)],
+@pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 14,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64),
+                                   (16384, 64), (65536, 16), (2, 16384)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_fp64_path(G, shape):
     n = int(np.prod(shape))
