@@ -26,6 +26,7 @@ struct MfmaParams {
   const cx<float>* tw4_lo;
   int tw4_bits;
   int tw4_shift;
+  int tw4_log2n;  // > 0: compute W_N^m (N = 2^tw4_log2n) instead of the table lookup
 };
 
 hipError_t gen_mfma_mats(void* out, int bf16, hipStream_t st);

@@ -349,7 +349,7 @@ __global__ void __launch_bounds__(256) mfma_dft256_kernel(MfmaParams p) {
     cx<float> z = v[it];
     if (p.tw4_hi) {
       const unsigned mi = (unsigned)((col0 + sc_) >> p.tw4_shift) * (unsigned)k;  // < N <= 2^24
-      z = cmul(z, tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_bits, tmask, mi));
+      z = cmul(z, p.tw4_log2n > 0 ? tw_pow2f(mi, p.tw4_log2n) : tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_bits, tmask, mi));
     }
     if (p.swap_out) z = swapri(z);
     st_elem<float>(p.out, nullptr, 1, ob + (idx_t)(col0 + sc_) * p.out_cs + (idx_t)k * p.out_ks, z, p.out_nt);

@@ -77,6 +77,7 @@ struct PassParams {
   int tw4_lo_bits;
   int tw4_col_shift;      // n2 = column >> shift
   int tw4_blo;            // 1: n2 = low batch index instead (columns of one n2 are the c index)
+  int tw4_log2n;          // > 0 (FP32): compute W_N^m, N = 2^tw4_log2n, instead of the table lookup
   const cx<T>* real_tw;   // W_{2L}^k, k < L (EPI_C2R4: W_{2h}^k, k < h)
   idx_t real_h;           // EPI_C2R4: h, the number of complex points
   // Byte extents of the in / out regions of one batch entry (fast path only).
@@ -598,7 +599,15 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           static_for<GS>([&](auto ic) AFGPU_INLINE {
             constexpr int m = g + (E / GS) * decltype(ic)::value;
             const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
-            const cx<T> w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
+            cx<T> w;
+            if constexpr (std::is_same_v<T, float>) {
+              if (p.tw4_log2n > 0)
+                w = tw_pow2f(mi, p.tw4_log2n);
+              else
+                w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
+            } else {
+              w = tw4_lookup32(p.tw4_hi, p.tw4_lo, p.tw4_lo_bits, mask, mi);
+            }
             v[m] = cmul(v[m], w);
             if (active) store(pol, t + m * TPT, v[m]);
           });

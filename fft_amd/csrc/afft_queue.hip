@@ -239,6 +239,7 @@ __device__ __noinline__ void q_chain(const QArgs& a, const QTask& tk, int tid, c
     pa.tw4_hi = a.tab->tw4[tk.lg];
     pa.tw4_lo = a.tab->tw4[tk.lg] + (m >> ((tk.lg + 1) / 2));
     pa.tw4_lo_bits = (tk.lg + 1) / 2;
+    pa.tw4_log2n = tk.lg;  // the twiddle is computed, not read from the tables
     pa.tw4_col_shift = 0;
     pa.stage_tw = a.tab->stage[tk.lg - 8];
     switch (n1) {

@@ -50,6 +50,7 @@ struct Real4Args {
   const cx<T>* tw4_hi;   // W_N^e = hi[e >> bits] * (1 + lo[e & mask]) (pass A)
   const cx<T>* tw4_lo;
   int tw4_bits;
+  int tw4_log2n;         // > 0 (FP32): compute W_N^e, N = 2^tw4_log2n, instead of the table lookup
   int ct, sv, hx;        // the compiled shape to launch (tile, stage schedule, half exchange)
   idx_t rs;              // c2r: entries per row of the scratch Dt[j2][k1] (>= n1/2 + 1)
 };

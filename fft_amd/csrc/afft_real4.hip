@@ -35,6 +35,16 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned id, unsigned n) {
   return x * per + (x < rem ? x : rem) + id / 8;
 }
 
+// The four-step twiddle W_N^e: computed (FP32, tw4_log2n > 0) or from the
+// two-level table.
+template <typename T>
+__device__ __forceinline__ cx<T> real4_tw(const Real4Args<T>& a, unsigned mask, unsigned e) {
+  if constexpr (std::is_same_v<T, float>) {
+    if (a.tw4_log2n > 0) return tw_pow2f(e, a.tw4_log2n);
+  }
+  return tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e);
+}
+
 template <typename T, int L, int CT, int SV, bool HX>
 struct R4AShape {
   using PS = PassShape<L, CT, SV>;
@@ -129,8 +139,8 @@ __global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T,
       cx<T> A = {s.x * h, s.y * h};
       cx<T> B = mul_neg_i(cx<T>{d.x * h, d.y * h});
       const unsigned e0 = (unsigned)(2 * col) * (unsigned)k1;
-      A = cmul(A, tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e0));
-      B = cmul(B, tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e0 + (unsigned)k1));
+      A = cmul(A, real4_tw(a, mask, e0));
+      B = cmul(B, real4_tw(a, mask, e0 + (unsigned)k1));
       cx<T>* row = dst + (idx_t)k1 * a.n2;
       row[0] = A;
       row[1] = B;
@@ -280,7 +290,7 @@ __global__ void __launch_bounds__((R4AShape<T, L, CT, SV, HX>::NT), (R4AShape<T,
     static_for<E>([&](auto mc) AFGPU_INLINE {
       constexpr int m = decltype(mc)::value;
       const int j2 = t + m * TPT;
-      const cx<T> wt = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)j2 * (unsigned)k1);
+      const cx<T> wt = real4_tw(a, mask, (unsigned)j2 * (unsigned)k1);
       dst[(idx_t)j2 * a.rs] = cmulc(swapri(v[m]), wt);
     });
   }
