@@ -10,8 +10,10 @@ Workloads: c2c_2p20 (64 x 2^20), c2c_2p16 (1024 x 2^16), c2c_2p12, r2c_2p22
 (16 x 2^22), r2c_2p20 (64 x 2^20; r2c_2p20_tr with the complex four-step and
 its real pass), c2r_2p20 (64 x 2^20), c2r_2p22 (16 x 2^22), c2c_2p25 (2 x 2^25,
 three passes), c2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
-and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), copy (512
-MiB device copy, the counter calibration), queue (10,000 uneven jobs).
+and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), the short
+rows c2c_2p4 (2^21 x 16) and r2c_2p5 (2^20 x 32), the longest one-pass rows
+c2c_2p15 (1024 x 2^15) and c2r_2p16 (512 x 2^16), copy (512 MiB device copy,
+the counter calibration), queue (10,000 uneven jobs).
 """
 from __future__ import annotations
 
@@ -75,6 +77,10 @@ def main():
             "c2c_2p25": ("c", (1 << 25,), 2),
             "c2c_2p16_4096": ("c", (1 << 16,), 4096),
             "c2c_2p16_4096_mfma": ("c", (1 << 16,), 4096),
+            "c2c_2p4": ("c", (16,), 1 << 21),
+            "r2c_2p5": ("r", (32,), 1 << 20),
+            "c2c_2p15": ("c", (1 << 15,), 1024),
+            "c2r_2p16": ("ri", (1 << 16,), 512),
         }
         kind, dims, batch = shapes[work]
         n = int(np.prod(dims))
