@@ -132,14 +132,15 @@ __device__ __forceinline__ cx<T> twc(cx<T> a) {
   }
 }
 
-// W_N^m = exp(-2 pi i m / N) in FP32 for N = 2^lg (3 <= lg <= 24, any 32-bit m)
+// W_N^m = exp(-2 pi i m / N) in FP32 for N = 2^lg (3 <= lg <= 27, any 32-bit m)
 // without a table.  The index is reduced exactly in integers: quadrant q =
 // m / (N/4), position r in the quadrant, folded to u = min(r, N/4 - r) <= N/8
 // so that the angle x = 2 pi u / N lies in [0, pi/4].  sin x and cos x come
 // from their Taylor series through x^9 and x^10 (the first omitted terms are
 // below 2e-9 there); the fold swaps them, and the quadrant rotates the pair
 // by q quarter turns.  Over every m of N = 2^20 this is within 9.3e-8 of the
-// exact value (rms 2.7e-8), as accurate as the two-level table product.
+// exact value (rms 2.7e-8), as accurate as the two-level table product; the
+// same holds up to N = 2^27 (u <= 2^24 stays exact in a float).
 __device__ __forceinline__ cx<float> tw_pow2f(unsigned m, int lg) {
   const unsigned q = m >> (lg - 2);
   const unsigned r = m & ((1u << (lg - 2)) - 1u);

@@ -598,7 +598,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           constexpr int g = decltype(gc)::value;
           static_for<GS>([&](auto ic) AFGPU_INLINE {
             constexpr int m = g + (E / GS) * decltype(ic)::value;
-            const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^24
+            const unsigned mi = n2 * (unsigned)(t + m * TPT);  // < N <= 2^28
             cx<T> w;
             if constexpr (std::is_same_v<T, float>) {
               if (p.tw4_log2n > 0)

@@ -1602,12 +1602,13 @@ void resolve(const Side& s, const Bufs<T>& B, idx_t start, const T*& re, const T
 
 // Kernel arguments of an FFT pass over user transforms [start, start+cnt);
 // log2 N where the kernels compute the pass's four-step twiddle W_N instead of
-// reading its tables (FP32, power-of-two N <= 2^24), else 0.  64 x 2^20 pass
-// A: 0.285 -> 0.22 ms per direction with both table loads per element gone.
+// reading its tables (FP32, power-of-two N <= 2^27, whose folded index N/8
+// is exact in a float), else 0.  64 x 2^20 pass A: 0.285 -> 0.22 ms per
+// direction with both table loads per element gone.
 template <typename T>
 int tw4_log2n(const PassRec& r) {
   const idx_t n = r.tw4_n;
-  return (sizeof(T) == 4 && n >= 8 && n <= ((idx_t)1 << 24) && (n & (n - 1)) == 0) ? ilog2(n) : 0;
+  return (sizeof(T) == 4 && n >= 8 && n <= ((idx_t)1 << 27) && (n & (n - 1)) == 0) ? ilog2(n) : 0;
 }
 
 // `fast` tells whether the buffer-resource path applies.
