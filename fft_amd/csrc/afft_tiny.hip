@@ -1,11 +1,11 @@
 // afft_tiny.hip -- rows of L <= 32 complex points, one row per thread.
 //
-// A row pass whose rows are this short gives each thread a whole row (all L
-// elements in registers), so a wave's load of element j touches 64 rows, 8
-// bytes each, L*8 bytes apart: 16-point rows ran at 0.37 TB/s on the general
-// row kernel, 4.0 TB/s here.  Here a work-group stages its R rows through LDS: the loads and stores run over
-// the R rows' elements in order (consecutive lanes, consecutive addresses),
-// and each thread then reads its row from LDS, runs the L-point DFT in
+// The general row kernel gives each thread of a short row pass a whole row
+// read in place, so a wave's load of element j touches 64 rows, 8 bytes each,
+// L*8 bytes apart (16-point rows: 0.37 TB/s).  Here a work-group stages its R
+// rows through LDS instead: the loads and stores run over the R rows' elements
+// in order (consecutive lanes, consecutive addresses; 16-point rows: 4.6
+// TB/s), and each thread reads its row from LDS, runs the L-point DFT in
 // registers (compile-time butterflies and twiddles) with the real pass of
 // r2c / c2r beside it, and writes its row back to LDS for the store.
 //   c2c: X = DFT(x) (inverse: re / im exchanged around it, on each side as
