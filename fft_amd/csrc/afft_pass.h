@@ -533,56 +533,72 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
   };
   with_out_pol([&](auto pol) AFGPU_INLINE {
     if constexpr (EPI == EPI_R2C) {
-      // the partner Z[L-k] of each bin crosses the LDS as a whole entry, or
-      // (HX) as its real part, then its imaginary part
-      T pre[HX ? E : 1];
+      // X[k] from Z[k] and its partner Z[L-k]; the first bin also gives X[L]
+      auto finish = [&](auto mc, cx<T> s1) AFGPU_INLINE {
+        constexpr int m = decltype(mc)::value;
+        const int k = t + m * TPT;
+        const cx<T> s0 = v[m];
+        cx<T> x;
+        if (k == 0) {
+          x = {s0.x + s0.y, T(0)};
+          store(pol, L, cx<T>{s0.x - s0.y, T(0)});
+        } else {
+          const cx<T> s1c = conjg(s1);
+          const cx<T> w = p.real_tw[k];
+          const T h = T(0.5);
+          const cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
+          const cx<T> bw = cmul(s0 - s1c, w);
+          x = a + mul_neg_i(cx<T>{bw.x * h, bw.y * h});
+        }
+        store(pol, k, x);
+      };
       if constexpr (HX) {
-        static_for<E>([&](auto mc) AFGPU_INLINE {
-          constexpr int m = decltype(mc)::value;
-          latf(t + m * TPT) = v[m].x;
+        // half-LDS tiles (one row): two rounds.  The partners of the lower
+        // bins k < L/2 (registers m < E/2) are upper bins and the other way
+        // round, so each round stages one half of the row as whole complex
+        // entries -- L/2 of them fill the half-LDS tile -- and no partner is
+        // held in registers
+        static_assert(CT == 1 && !COLMAP, "the two-round r2c epilogue is for single-row half-LDS tiles");
+        static_assert(E % 2 == 0, "the two rounds split the registers in halves");
+        cx<T>* __restrict__ half_row = reinterpret_cast<cx<T>*>(ldsf);
+        constexpr int MH = E / 2;
+        static_for<E - MH>([&](auto ic) AFGPU_INLINE {
+          constexpr int m = MH + decltype(ic)::value;
+          half_row[lds_pad(t + m * TPT - L / 2)] = v[m];
         });
         __syncthreads();
-        static_for<E>([&](auto mc) AFGPU_INLINE {
+        if (active) {
+          static_for<MH>([&](auto mc) AFGPU_INLINE {
+            const int k = t + decltype(mc)::value * TPT;
+            finish(mc, k == 0 ? v[0] : half_row[lds_pad(L - k - L / 2)]);
+          });
+        }
+        __syncthreads();
+        static_for<MH>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
-          pre[m] = latf((L - (t + m * TPT)) & (L - 1));
+          half_row[lds_pad(t + m * TPT)] = v[m];
         });
         __syncthreads();
-        static_for<E>([&](auto mc) AFGPU_INLINE {
-          constexpr int m = decltype(mc)::value;
-          latf(t + m * TPT) = v[m].y;
-        });
+        if (active) {
+          static_for<E - MH>([&](auto ic) AFGPU_INLINE {
+            constexpr int m = MH + decltype(ic)::value;
+            const int k = t + m * TPT;
+            // bin L/2 is its own partner
+            finish(std::integral_constant<int, m>{}, k == L / 2 ? v[m] : half_row[lds_pad(L - k)]);
+          });
+        }
       } else {
         static_for<E>([&](auto mc) AFGPU_INLINE {
           constexpr int m = decltype(mc)::value;
           lat(t + m * TPT) = v[m];
         });
-      }
-      __syncthreads();
-      if (active) {
-        static_for<E>([&](auto mc) AFGPU_INLINE {
-          constexpr int m = decltype(mc)::value;
-          const int k = t + m * TPT;
-          cx<T> s0 = v[m];
-          cx<T> x;
-          if (k == 0) {
-            x = {s0.x + s0.y, T(0)};
-            store(pol, L, cx<T>{s0.x - s0.y, T(0)});
-          } else {
-            cx<T> s1c;
-            if constexpr (HX)
-              s1c = {pre[m], -latf(L - k)};
-            else
-              s1c = conjg(lat(L - k));
-            cx<T> w = p.real_tw[k];
-            const T h = T(0.5);
-            cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
-            cx<T> d = s0 - s1c;
-            cx<T> bw = cmul(d, w);
-            cx<T> bb = {bw.x * h, bw.y * h};
-            x = a + mul_neg_i(bb);
-          }
-          store(pol, k, x);
-        });
+        __syncthreads();
+        if (active) {
+          static_for<E>([&](auto mc) AFGPU_INLINE {
+            const int k = t + decltype(mc)::value * TPT;
+            finish(mc, k == 0 ? v[0] : lat(L - k));
+          });
+        }
       }
     } else {
       if constexpr ((EPI == EPI_TW4 || EPI == EPI_C2R4) && !STORE_T) {
