@@ -140,6 +140,7 @@ struct RealIColPairArgs {
   const cx<T>* tw4_hi;    // W_h^e = tw4_hi[e >> tw4_bits] * tw4_lo[e & mask]
   const cx<T>* tw4_lo;
   int tw4_bits;
+  int tw4_log2n;          // > 0 (FP32): compute W_h^e, h = 2^tw4_log2n, instead of the table lookup
 };
 
 // all = false: only the row lengths where the fused pass is the default

@@ -1824,6 +1824,7 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.tw4_hi = (const cx<T>*)r.tw4_hi;
     a.tw4_lo = (const cx<T>*)r.tw4_lo;
     a.tw4_bits = r.tw4_bits;
+    a.tw4_log2n = tw4_log2n<T>(r);
     HIP_OK(real_icolpair<T>(a, st));
     return true;
   }

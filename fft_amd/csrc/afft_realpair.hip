@@ -706,7 +706,12 @@ __global__ void __launch_bounds__((PassShape<L, rcp_ct<T, L>(), 0>::NT)) real_ic
     static_for<E>([&](auto mc) AFGPU_INLINE {
       constexpr int mm = decltype(mc)::value;
       const int k1 = t + mm * TPT;
-      const cx<T> w = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, (unsigned)n2 * (unsigned)k1);
+      const unsigned e = (unsigned)n2 * (unsigned)k1;
+      cx<T> w;
+      if constexpr (std::is_same_v<T, float>)
+        w = a.tw4_log2n > 0 ? tw_pow2f(e, a.tw4_log2n) : tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e);
+      else
+        w = tw4_lookup32(a.tw4_hi, a.tw4_lo, a.tw4_bits, mask, e);
       dst[((idx_t)k1 * a.h2 + n2) * a.m] = cmul(v[mm], w);
     });
   };
