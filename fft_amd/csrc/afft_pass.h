@@ -335,6 +335,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, un
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
 
+// W_2L^k (k < L) of the real pass beside an L-point row DFT: computed in FP32
+// (one load per bin less), read from the table in FP64.
+template <typename T, int L>
+__device__ __forceinline__ cx<T> real_twiddle(const cx<T>* __restrict__ tab, int k) {
+  if constexpr (std::is_same_v<T, float> && L >= 4)
+    return tw_pow2f((unsigned)k, __builtin_ctz(2 * L));
+  else
+    return tab[k];
+}
+
 // A cache policy known only at run time (one branch per access).
 struct RtPol {
   int nt;
@@ -497,7 +507,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           if (k == 0) {
             z = {s0.x + s1.x, s0.x - s1.x};
           } else {
-            cx<T> w = p.real_tw[k];
+            cx<T> w = real_twiddle<T, L>(p.real_tw, k);
             cx<T> s1c = conjg(s1);
             cx<T> a = s0 + s1c;
             cx<T> bb = cmulc(s1c - s0, w);
@@ -544,7 +554,7 @@ __device__ __forceinline__ void fft_tile(const PassParams<T>& p, int tile, int t
           store(pol, L, cx<T>{s0.x - s0.y, T(0)});
         } else {
           const cx<T> s1c = conjg(s1);
-          const cx<T> w = p.real_tw[k];
+          const cx<T> w = real_twiddle<T, L>(p.real_tw, k);
           const T h = T(0.5);
           const cx<T> a = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
           const cx<T> bw = cmul(s0 - s1c, w);

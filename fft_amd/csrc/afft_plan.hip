@@ -1860,7 +1860,6 @@ bool run_pass(const PassRec& r, const Bufs<T>& B, idx_t start, idx_t cnt, hipStr
     a.out_rs = r.out.s_c;
     a.swap_in = r.swap_in;
     a.swap_out = r.swap_out;
-    a.real_tw = (const cx<T>*)r.real_tw;
     HIP_OK(tiny_rows<T>(a, r.L, r.epi, r.CT, st));
     return true;
   }

@@ -25,7 +25,6 @@ struct TinyArgs {
   idx_t out_rs;
   int swap_in;      // exchange re / im of the loaded rows (inverse c2c: around a forward DFT)
   int swap_out;     // exchange re / im of the stored rows
-  const cx<T>* real_tw;  // W_2L^k, k < L (r2c / c2r)
 };
 
 // Default rows per work-group (one per thread: 64, 128 or 256; 0 where the

@@ -6,8 +6,9 @@
 // rows through LDS instead: the loads and stores run over the R rows' elements
 // in order (consecutive lanes, consecutive addresses; 16-point rows: 4.6
 // TB/s), and each thread reads its row from LDS, runs the L-point DFT in
-// registers (compile-time butterflies and twiddles) with the real pass of
-// r2c / c2r beside it, and writes its row back to LDS for the store.
+// registers (compile-time butterflies and twiddles, the real pass's W_2L^k
+// included) with the real pass of r2c / c2r beside it, and writes its row back
+// to LDS for the store.
 //   c2c: X = DFT(x) (inverse: re / im exchanged around it, on each side as
 //        the pass asks);
 //   r2c: the reals as L pairs z[j] = x[2j] + i x[2j+1], Z = DFT(z), then afft's
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(R) tiny_rows_kernel(TinyArgs<T> a) {
           z = {s0.x + s1.x, s0.x - s1.x};
         } else {
           const cx<T> s1c = conjg(s1);
-          z = s0 + s1c + mul_neg_i(cmulc(s1c - s0, a.real_tw[k]));
+          z = s0 + s1c + mul_neg_i(twc<T, 2 * L, -k>(s1c - s0));
         }
         v[k] = swapri(z);
       });
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(R) tiny_rows_kernel(TinyArgs<T> a) {
         } else {
           const cx<T> s1c = conjg(v[L - k]);
           const cx<T> av = {(s0.x + s1c.x) * h, (s0.y + s1c.y) * h};
-          const cx<T> bw = cmul(s0 - s1c, a.real_tw[k]);
+          const cx<T> bw = twc<T, 2 * L, k>(s0 - s1c);
           row[k] = av + mul_neg_i(cx<T>{bw.x * h, bw.y * h});
         }
       });
