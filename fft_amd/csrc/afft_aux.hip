@@ -126,7 +126,7 @@ __global__ void real_rows_kernel(RealRowsArgs<T> a) {
     return;
   }
   cx<T> s0 = ld(k), s1 = ld(h - k);
-  cx<T> w = a.real_tw[k];
+  cx<T> w = real_tw_pow2(a.real_tw, k, h);
   cx<T> s1c = conjg(s1);
   cx<T> d0, d1;
   if (INV) {

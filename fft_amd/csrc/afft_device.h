@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include <utility>
 
 namespace afgpu {
@@ -163,6 +164,17 @@ __device__ __forceinline__ cx<float> tw_pow2f(unsigned m, int lg) {
     b = -b;
   }
   return {a, -b};
+}
+
+// W_2h^k (k < h, h a power of two) of a real pass: computed in FP32 while
+// 2h <= 2^27, read from the table (W_2h^k at entry k) otherwise.
+template <typename T>
+__device__ __forceinline__ cx<T> real_tw_pow2(const cx<T>* __restrict__ tab, idx_t k, idx_t h) {
+  if constexpr (std::is_same_v<T, float>) {
+    const int lg = __builtin_ctzll((unsigned long long)h) + 1;
+    if (lg >= 3 && lg <= 27) return tw_pow2f((unsigned)k, lg);
+  }
+  return tab[k];
 }
 
 // In-place, natural-order forward DFT of R points held in registers.

@@ -556,7 +556,7 @@ __global__ void __launch_bounds__((PassShape<L, rcp_ct<T, L>(), 0>::NT)) real_co
     if (mine) {
       const idx_t k = (idx_t)ka + (idx_t)a.h1 * k2;
       const cx<T> s0 = va[mm];
-      const cx<T> w = a.real_tw[k];
+      const cx<T> w = real_tw_pow2(a.real_tw, k, h);
       const cx<T> s1c = conjg(s1);
       const T hf = T(0.5);
       const cx<T> aa = {(s0.x + s1c.x) * hf, (s0.y + s1c.y) * hf};
