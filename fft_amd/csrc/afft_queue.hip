@@ -63,7 +63,7 @@ constexpr int kQThreads = 256;
 constexpr int kQMaxLog2 = 16;
 constexpr int kBigN2 = 256;
 constexpr int kRealTile = 2048;  // bin pairs per real-pass tile
-constexpr int kLag = 8;          // default jobs between consecutive steps of one chain (per XCD list)
+constexpr int kLag = 32;         // default jobs between consecutive steps of one chain (per XCD list)
 constexpr int kXcd = 8;          // XCDs of an MI355X: one chain list each
 constexpr int kCtr = 16;         // ints between counters (one 64-byte line each)
 
@@ -794,7 +794,7 @@ int afft_gpu_queue_tune(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t njo
   int bg = keep_groups, bl = keep_lag;
   bool failed = false;
   for (int groups : {4, 2}) {
-    for (int lag : {kLag, 2, 32}) {
+    for (int lag : {kLag, 8, 2}) {
       if (afft_gpu_queue_set_groups_per_cu(qh, groups) != 0 || afft_gpu_queue_set_lag(qh, lag) != 0 ||
           afft_gpu_queue_prepare(qh, jobs, njobs) < 0 || afft_gpu_queue_launch(qh, stream) != 0) {
         failed = true;
