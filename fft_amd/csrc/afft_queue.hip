@@ -23,8 +23,8 @@
 // own list, so every tile of a job runs under one L2.  Within a list the
 // chains are interleaved with a lag of kLag jobs between a step and the next
 // (step s of job j is listed beside step 0 of job j + s*kLag), so a dependent
-// tile is normally popped long after the tiles it waits for, and the scratch
-// of the jobs in flight stays small.  The GROUP tasks form one shared list,
+// tile is normally popped long after the tiles it waits for (each chain job
+// has a scratch slot of its own, so the lag changes only the order).  The GROUP tasks form one shared list,
 // longest first, that every work-group drains once its own list is empty.
 // Device side: a persistent kernel; lists are popped through atomic counters.
 // A task only ever waits for tasks listed (hence popped) before it in its own
@@ -63,7 +63,7 @@ constexpr int kQThreads = 256;
 constexpr int kQMaxLog2 = 16;
 constexpr int kBigN2 = 256;
 constexpr int kRealTile = 2048;  // bin pairs per real-pass tile
-constexpr int kLag = 32;         // default jobs between consecutive steps of one chain (per XCD list)
+constexpr int kLag = 128;        // default jobs between consecutive steps of one chain (per XCD list)
 constexpr int kXcd = 8;          // XCDs of an MI355X: one chain list each
 constexpr int kCtr = 16;         // ints between counters (one 64-byte line each)
 
@@ -794,7 +794,7 @@ int afft_gpu_queue_tune(afft_gpu_queue* qh, const afft_gpu_job* jobs, size_t njo
   int bg = keep_groups, bl = keep_lag;
   bool failed = false;
   for (int groups : {4, 2}) {
-    for (int lag : {kLag, 8, 2}) {
+    for (int lag : {kLag, 32, 8}) {
       if (afft_gpu_queue_set_groups_per_cu(qh, groups) != 0 || afft_gpu_queue_set_lag(qh, lag) != 0 ||
           afft_gpu_queue_prepare(qh, jobs, njobs) < 0 || afft_gpu_queue_launch(qh, stream) != 0) {
         failed = true;

@@ -252,7 +252,7 @@ int afft_gpu_queue_set_spin_limit(afft_gpu_queue* q, long long limit);
  * lane, some spills) or 2 (registers for the largest tiles, half the tiles in
  * flight). */
 int afft_gpu_queue_set_groups_per_cu(afft_gpu_queue* q, int groups);
-/* Jobs between consecutive steps of one long job in a task list (default 32;
+/* Jobs between consecutive steps of one long job in a task list (default 128;
  * 0..1024): larger lags let more of a chain's tiles finish before a dependent
  * tile is popped, smaller ones keep a job's data in cache.  Takes effect at
  * the next prepare. */
