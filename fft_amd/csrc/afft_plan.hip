@@ -751,8 +751,20 @@ struct Builder {
   // transposing real pass reads Z[k1 + n1*k2] = Y[k1][k2] and writes the h+1
   // bins in natural order (one kernel for the transposition and the real
   // post-pass instead of a transposed store and a separate real pass).
+  // Row length of r2c_fourstep_tr: its pass B stores rows (not transposed),
+  // so long rows cost nothing there, while 4096-point columns fit only four
+  // per pass-A tile: 1024-point columns where the rows fit one row pass, else
+  // the 1-D default (FP32 4 x r2c 2^24: 0.742 ms as 4096 x 2048, 0.428 as
+  // 2048 x 4096, 0.410 as 1024 x 8192).
+  idx_t r2c_tr_row_len(idx_t h) const {
+    if (P.var.row_log2 > 0 || h < ((idx_t)1 << 21)) return row_len_1d(h);
+    if (h / 1024 <= row_max()) return h / 1024;
+    if (h / 2048 <= row_max()) return h / 2048;
+    return row_len_1d(h);
+  }
+
   void r2c_fourstep_tr(const Side& in, const Side& out, idx_t h) {
-    const idx_t n2 = row_len_1d(h);
+    const idx_t n2 = r2c_tr_row_len(h);
     const idx_t n1 = n2 >= 2 ? h / n2 : 0;
     if (n1 > max_col_len<T>() || n1 < 32) {
       set_err("transform length %lld is too long", (long long)(2 * h));
@@ -1344,7 +1356,7 @@ struct Builder {
           return ok;
         }
         if (r2c_real4(in, out, h)) return ok;
-        const idx_t n2 = row_len_1d(h);
+        const idx_t n2 = r2c_tr_row_len(h);
         if (n2 >= 2 && h / n2 >= 32 && h <= (idx_t)max_col_len<T>() * 4096) {
           r2c_fourstep_tr(in, out, h);
           return ok;

@@ -186,6 +186,21 @@ def test_plan_model_tiny_rows(kind):
         assert 13 not in [p.type for p in passes]
 
 
+def test_plan_model_r2c_transposing_real_pass_split():
+    # the complex-form 1-D r2c (pass A, pass B into a second scratch, the
+    # transposing real pass) takes 1024-point columns once h >= 2^21: its pass B
+    # stores rows, so long rows cost nothing there
+    err, info, passes = model_check("r", (1 << 22,), SP, 1, prec=64)
+    assert [(p.type, p.L, p.colmap) for p in passes] == [(0, 1024, 1), (0, 2048, 0), (3, 1, 0)]
+    assert passes[2].rr_m == 1024 and err < 1e-9, err
+    for lg, n2 in ((24, 8192), (25, 16384)):
+        info, passes = M.inspect("r", (1 << lg,), IL, 1)
+        assert [(p.type, p.L) for p in passes] == [(0, 1024), (0, n2), (3, 1)], (lg, [(p.type, p.L) for p in passes])
+    # an explicit row length keeps the variant's split
+    info, passes = M.inspect("r", (1 << 24,), IL, 1, variant={"row_log2": 11})
+    assert [(p.type, p.L) for p in passes] == [(0, 4096), (0, 2048), (3, 1)]
+
+
 @pytest.mark.parametrize("kind", ["r", "ri"])
 def test_plan_model_real_pass_in_rows(kind):
     # long real dim 0 with a 1024-point last axis: by default the real pass
