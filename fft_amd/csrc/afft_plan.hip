@@ -549,8 +549,10 @@ struct Builder {
       r->swap_out = swo;
       return;
     }
-    // 1-D lengths beyond two passes (n > 4096 * 4096): three passes
-    if (inner == 1 && P.var.row_log2 <= 0 && n > (idx_t)max_col_len<T>() * 4096) {
+    // 1-D lengths beyond 2048 x 2048: three passes.  Two passes would need
+    // 4096-point columns, four per tile (32-byte row segments): 8 x c2c 2^23
+    // took 1.17 ms and 4 x 2^24 1.73 ms as two passes
+    if (inner == 1 && P.var.row_log2 <= 0 && n > (idx_t)2048 * 2048) {
       fourstep3(in, out, outer_u, n, swi, swo, tmp_role);
       return;
     }
