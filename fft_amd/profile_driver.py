@@ -12,8 +12,9 @@ its real pass), c2r_2p20 (64 x 2^20), c2r_2p22 (16 x 2^22), c2c_2p25 (2 x 2^25,
 three passes), c2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
 and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), the short
 rows c2c_2p4 (2^21 x 16) and r2c_2p5 (2^20 x 32), the longest one-pass rows
-c2c_2p15 (1024 x 2^15) and c2r_2p16 (512 x 2^16), copy (512 MiB device copy,
-the counter calibration), queue (10,000 uneven jobs).
+c2c_2p15 (1024 x 2^15) and c2r_2p16 (512 x 2^16), the three-pass c2c_2p23
+(8 x 2^23) and c2c_2p24 (4 x 2^24), copy (512 MiB device copy, the counter
+calibration), queue (10,000 uneven jobs).
 """
 from __future__ import annotations
 
@@ -81,6 +82,8 @@ def main():
             "r2c_2p5": ("r", (32,), 1 << 20),
             "c2c_2p15": ("c", (1 << 15,), 1024),
             "c2r_2p16": ("ri", (1 << 16,), 512),
+            "c2c_2p23": ("c", (1 << 23,), 8),
+            "c2c_2p24": ("c", (1 << 24,), 4),
         }
         kind, dims, batch = shapes[work]
         n = int(np.prod(dims))
