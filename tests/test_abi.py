@@ -406,7 +406,7 @@ def test_plan_shapes_of_headline_configs():
 
 
 def test_plan_model_three_pass_fourstep():
-    # 1-D lengths above 4096 * 4096: the four-step n = p * M whose row DFTs
+    # 1-D lengths above 2048 * 2048: the four-step n = p * M whose row DFTs
     # (M = q * 2048) are a four-step too -- pass 1 p-point columns, pass 2
     # q-point columns in place, pass 3 2048-point rows storing bin
     # kp + p*(kq + q*k2); r2c / c2r put the real pass beside it
@@ -420,6 +420,15 @@ def test_plan_model_three_pass_fourstep():
     assert [(p.type, p.L) for p in passes] == [(1, 1), (0, 128), (0, 128), (0, 2048)]
     info, passes = M.inspect("r", (1 << 28,), SP, 1)
     assert [(p.type, p.L) for p in passes] == [(0, 256), (0, 256), (0, 2048), (1, 1)]
+    # from 2^23 on (a two-pass split would need 4096-point columns): FP32 and
+    # FP64, executed by the model for 2^23 in FP64 with a batch of 2
+    err, info, passes = model_check("c", (1 << 23,), SP, 2, prec=64)
+    assert [(p.type, p.L, p.colmap) for p in passes] == [(0, 64, 1), (0, 64, 1), (0, 2048, 0)], passes
+    assert err < 1e-9, err
+    info, passes = M.inspect("ci", (1 << 24,), IL, 1)
+    assert [(p.type, p.L) for p in passes] == [(0, 128), (0, 64), (0, 2048)]
+    info, passes = M.inspect("c", (1 << 22,), IL, 1)
+    assert [(p.type, p.L) for p in passes] == [(0, 2048), (0, 2048)]
 
 
 def test_removed_pipeline_variant_is_refused():

@@ -213,6 +213,19 @@ def test_fp64_path(G, shape):
     assert O.rel_l2(w, O.irfft(h, shape)) <= bound64(n)
 
 
+@pytest.mark.parametrize("kind", ["c", "ci"])
+def test_fp64_three_pass(G, kind):
+    """FP64 1-D c2c of 2^23 points (three-pass four-step) against the FP64
+    oracle, both formats."""
+    n = 1 << 23
+    x = O.complex_input(71, (n,)).astype(np.complex128)
+    ref = O.fft(x, inverse=kind == "ci")
+    for fmt in (G.IL, G.SP):
+        say("fp64 three-pass", kind, fmt)
+        y = G.gpu_c2c(x, kind == "ci", fmt, np.float64)
+        assert O.rel_l2(y, ref) <= bound64(n), (kind, fmt, O.rel_l2(y, ref))
+
+
 def test_roundtrip_scales_by_n(G):
     for n in (1 << 12, 1 << 20):
         say("roundtrip", n)
