@@ -451,13 +451,13 @@ def main():
         rows = {}
         xs = torch.rand(1 << 26, device=dev, generator=g, dtype=torch.float32)
         ys = torch.empty(1 << 26, device=dev, dtype=torch.float32)
-        for name, kind, n, b in (("c2c_2^4_x2^21", "c", 16, 1 << 21), ("r2c_2^5_x2^20", "r", 32, 1 << 20),
-                                 ("c2c_2^15_x1024", "c", 1 << 15, 1024), ("c2r_2^16_x512", "ri", 1 << 16, 512)):
-            ps = Plan(kind, (n,), b)
+        for name, kind, nr_, b in (("c2c_2^4_x2^21", "c", 16, 1 << 21), ("r2c_2^5_x2^20", "r", 32, 1 << 20),
+                                   ("c2c_2^15_x1024", "c", 1 << 15, 1024), ("c2r_2^16_x512", "ri", 1 << 16, 512)):
+            ps = Plan(kind, (nr_,), b)
             ps(xs, None, ys, None, stream)
             mss = time_median(torch, stream, lambda: ps(xs, None, ys, None, stream), 10)
-            m = n // 2 + 1
-            byts = 16 * n * b if kind == "c" else (4 * n + 8 * m) * b
+            m = nr_ // 2 + 1
+            byts = 16 * nr_ * b if kind == "c" else (4 * nr_ + 8 * m) * b
             rows[name] = {"ms": round(mss, 4), "GBps": round(byts / (mss * 1e-3) / 1e9, 1),
                           "pass": ps.describe()[0].split(" in(")[0]}
             ps.close()
