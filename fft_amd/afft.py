@@ -260,8 +260,9 @@ class Plan:
 
     def set_variant(self, **fields):
         """Rebuilds the plan with the given variant fields (others: default;
-        stream_io defaults to 1)."""
-        v = Variant(stream_io=1)
+        stream_io defaults to 2: streaming I/O except for N-D plans whose
+        arrays hold at most 128 MiB per side)."""
+        v = Variant(stream_io=2)
         for k, val in fields.items():
             setattr(v, k, int(val))
         if _lib.lib().afft_gpu_plan_set_variant(self.state, ctypes.byref(v)) != 0:

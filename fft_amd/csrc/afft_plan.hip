@@ -146,7 +146,8 @@ constexpr idx_t kReal4C2rMax = (idx_t)1 << 22;
 struct Variant {
   int col_tile = 0;   // columns per group of column passes with L >= 512 (0 = default)
   int rowt_tile = 0;  // rows per group of transposed-store passes (0 = default)
-  int stream_io = 1;  // non-temporal loads of src / stores of dst (keeps the MALL for scratch)
+  int stream_io = 2;  // non-temporal loads of src / stores of dst (keeps the MALL for scratch):
+                      // 1 always, 0 never, 2 (default) unless an N-D plan's arrays fit the MALL
   int xcd_order = 0;  // XCD-contiguous tile order
   idx_t chunk = 0;    // transforms per pass group (0 = default)
   int sched = 0;      // 1 / 0 (default): radix-32 schedules where compiled, -1: never
@@ -1231,6 +1232,18 @@ struct Builder {
     P.npass = 0;
     P.scr_per_user[0] = P.scr_per_user[1] = 0;
     bool r = build_passes(kind, format, d, nd);
+    // stream_io 2: an N-D plan (its outer axes re-read dst) whose arrays hold at
+    // most 128 MiB per side keeps cached I/O, so input and output share the
+    // 256 MB MALL with the passes between them (measured: r2c 256^3 0.165 ->
+    // 0.137 ms, c2c 1024^2 x 16 0.098 -> 0.092, c2c 256^3 0.131 -> 0.127);
+    // 256 MiB arrays stream (c2c 4096^2 x 2 0.277 streaming, 0.293 cached)
+    bool nd_reread = false;
+    for (int i = 0; i < P.npass && !nd_reread; i++)
+      for (int j = i + 1; j < P.npass; j++) nd_reread = nd_reread || (P.pass[j].in.role == P.pass[i].out.role &&
+                                                                     P.pass[i].out.role == R_DST);
+    double side_bytes = (double)P.batch * (double)sizeof(T) * (kind < 2 ? 2.0 : 1.0);
+    for (int k = 0; k < nd; k++) side_bytes *= (double)d[k];
+    const int sio = P.var.stream_io == 2 ? !(nd_reread && side_bytes <= 128.0 * 1048576.0) : P.var.stream_io;
     for (int i = 0; i < P.npass; i++) {
       PassRec& q = P.pass[i];
       // Streaming (non-temporal) accesses for the caller's input and for the
@@ -1239,8 +1252,8 @@ struct Builder {
       // through the caches, so the array can stay in the MALL in between.
       bool reread = false;
       for (int j = i + 1; j < P.npass; j++) reread = reread || P.pass[j].in.role == q.out.role;
-      q.in_nt = P.var.stream_io && q.in.role == R_SRC;
-      q.out_nt = P.var.stream_io && q.out.role == R_DST && !reread;
+      q.in_nt = sio && q.in.role == R_SRC;
+      q.out_nt = sio && q.out.role == R_DST && !reread;
       // the real-input four-step's row passes store (r2c) or load (c2r) the
       // spectrum in runs of a tile's rows; a transform's N/2 + 1 bins rarely
       // start on a line boundary, so neighbouring tiles share lines: cached
@@ -2424,8 +2437,14 @@ bool tune_impl(Plan& P, const void* sre, const void* sim, void* dre, void* dim, 
     if (!rebuild<T>(P)) return false;
     base_chunk = P.chunk;
   }
-  // the XCD order and streaming I/O stay manual knobs: neither has lost
-  // (streaming) or won (XCD order) on a measured shape
+  // streaming I/O forced on and off, and the XCD-contiguous tile order, on the
+  // default shape (each moved N-D transforms by a few per cent either way)
+  for (int sx : {0, 1, 2}) {
+    Variant v;
+    if (sx < 2) v.stream_io = sx;
+    else v.xcd_order = 1;
+    cands.push_back(v);
+  }
   for (int col : {0, 8, 32})
     for (int rowt : {0, 4, 16})
       for (int hl : {0, -1})
@@ -3017,6 +3036,7 @@ int afft_gpu_plan_inspect_variant(int precision, int kind, size_t ndim, const si
     o.tw4_shift = r.tw4_shift;
     o.tw4_blo = r.tw4_blo;
     o.sv = r.sv;
+    o.nt_io = (r.in_nt ? 1 : 0) | (r.out_nt ? 2 : 0);
     o.rr_h = r.rr_h;
     o.rr_m = r.rr_m;
     o.rr_inv = r.rr_inv;

@@ -117,7 +117,11 @@ typedef struct
                         for the 256-, 512- and 1024-point and 64 for the 256-point
                         four-step columns) */
   int32_t rowt_tile; /* rows per work-group of transposed-store row passes (4, 8 or 16) */
-  int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output */
+  int32_t stream_io; /* 1: non-temporal loads of the caller's input, stores of the output;
+                        0: cached; 2 (the default of a plan created without a variant):
+                        cached for N-D plans whose arrays hold at most 128 MiB per side
+                        (input, output and the passes between them share the MALL),
+                        non-temporal otherwise */
   int32_t xcd_order; /* 1: hand each XCD a contiguous run of tiles */
   int64_t chunk;     /* transforms per pass group of four-step plans */
   int32_t sched;     /* 1 or 0 (default): radix-32 stage schedules for the long passes
@@ -319,7 +323,7 @@ typedef struct
   int32_t tw4_blo; /* 1: the four-step twiddle's column index n2 is the low batch index
                       (b % n_blo, N = L * n_blo) instead of c >> tw4_shift */
   int32_t sv;      /* stage schedule of the pass kernel (0: default radices, 1: radix-32) */
-  int32_t pad_;
+  int32_t nt_io;   /* bit 0: non-temporal loads of the pass input, bit 1: non-temporal stores */
 } afft_gpu_pass_info;
 
 typedef struct

@@ -34,7 +34,7 @@ class PassInfo(ctypes.Structure):
                 ("swap_in", ctypes.c_int32), ("swap_out", ctypes.c_int32), ("tw4_bits", ctypes.c_int32),
                 ("tw4_shift", ctypes.c_int32), ("rr_h", ctypes.c_int64), ("rr_m", ctypes.c_int64),
                 ("rr_inv", ctypes.c_int32), ("tw4_blo", ctypes.c_int32), ("sv", ctypes.c_int32),
-                ("pad_", ctypes.c_int32)]
+                ("nt_io", ctypes.c_int32)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -47,7 +47,7 @@ KIND = {"c": 0, "ci": 1, "r": 2, "ri": 3}
 
 def inspect(kind: str, dims, fmt: int, batch: int = 1, prec: int = 32, variant: dict | None = None):
     """Pass list of the plan (variant: afft_gpu_variant fields, others 0 /
-    stream_io 1 as afft.Plan.set_variant sets them)."""
+    stream_io 2 as afft.Plan.set_variant sets them)."""
     from fft_amd.afft import Variant
     L = _lib.lib()
     f = L.afft_gpu_plan_inspect_variant
@@ -59,7 +59,7 @@ def inspect(kind: str, dims, fmt: int, batch: int = 1, prec: int = 32, variant: 
     passes = (PassInfo * 200)()
     v = None
     if variant is not None:
-        v = Variant(stream_io=1)
+        v = Variant(stream_io=2)
         for k, val in variant.items():
             setattr(v, k, int(val))
     n = f(prec, KIND[kind], nd, arr, fmt, batch, ctypes.byref(v) if v is not None else None, ctypes.byref(info),

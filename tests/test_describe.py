@@ -30,7 +30,7 @@ def _describe(prec, kind, shape, fmt, batch, v, i, size=512):
     nd, arr = _lib.dims_array(shape)
     vv = None
     if v is not None:
-        vv = Variant(stream_io=1)
+        vv = Variant(stream_io=2)
         for k, val in v.items():
             setattr(vv, k, int(val))
     buf = ctypes.create_string_buffer(size)
