@@ -15,7 +15,7 @@ The JSON line also carries
   cpu_baseline  afft itself (oracle/_ref, built from the reference sources) on
                 this host's cores over a bounded sample of the same workload;
   extra         the rest of the afft configs measured the same way (r2c
-                16 x 2^22, c2c 256^3, r2c 8192^2, FP64 c2c 16 x 2^20, the
+                16 x 2^22, c2c and r2c 256^3, r2c 8192^2, FP64 c2c 16 x 2^20, the
                 device-copy ceiling),
                 the MFMA split-precision mode against the FP32-FMA plan on
                 4096 x 2^16 with 10^u amplitudes (time and error vs FP64),
@@ -421,6 +421,16 @@ def main():
         extra["c2c_256^3_ms"] = round(ms3, 4)
         extra["c2c_256^3_GFLOPs"] = round(flops_c2c(n3, 1) / (ms3 * 1e-3) / 1e9, 1)
         extra["c2c_256^3_hbm_frac"] = round(16 * n3 / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+        del x3, y3, p3
+        # r2c 256^3 (standard normal)
+        x3 = torch.randn(n3, device=dev, generator=g, dtype=torch.float32)
+        y3 = torch.empty(2 * 129 * 256 * 256, device=dev, dtype=torch.float32)
+        p3 = Plan("r", (256, 256, 256), 1)
+        p3(x3, None, y3, None, stream)
+        ms3 = time_median(torch, stream, lambda: p3(x3, None, y3, None, stream), 10)
+        extra["r2c_256^3_ms"] = round(ms3, 4)
+        extra["r2c_256^3_GFLOPs_2.5NlogN"] = round(2.5 * n3 * 24 / (ms3 * 1e-3) / 1e9, 1)
+        extra["r2c_256^3_hbm_frac"] = round((4 * n3 + 8 * 129 * 256 * 256) / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
         del x3, y3, p3
         # r2c 8192^2
         n2 = 8192 * 8192

@@ -20,4 +20,5 @@
 #define AFGPU_COL_CT4096 4
 #define AFGPU_WIDE_COL1024 1
 #define AFGPU_C2R4 1
+#define AFGPU_WIDE_REAL_COL 1
 #include "afft_variants.inc"

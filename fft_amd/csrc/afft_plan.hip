@@ -425,7 +425,11 @@ struct Builder {
     // 64 x 2^20 direction for two 16-column groups; they stay a variant,
     // col_tile = 32)
     const bool wide = want <= 0 && colmap && P.var.half_lds >= 0 && L != 1024;
-    if (pref_ct <= 0 && want <= 0 && colmap && L <= 128) pref_ct = 4 * deflt;
+    // (the dim-0 real column passes too: r2c 256^3 0.135 -> 0.080 ms, 128^3 x 8
+    // 0.124 -> 0.076, 256 x 512 x 256 0.320 -> 0.160; c2r 256^3 0.0885 -> 0.085,
+    // but c2r with 64-point columns 0.087 -> 0.092, so those keep the default
+    // tile; col_tile = 64 still selects the wide one)
+    if (pref_ct <= 0 && want <= 0 && colmap && L <= 128 && !(epi == EPI_C2R && L < 128)) pref_ct = 4 * deflt;
     struct Cand {
       int ct, hx_only;
     };

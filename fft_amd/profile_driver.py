@@ -9,7 +9,7 @@ under profiles/ (the exact command sits in each summary).
 Workloads: c2c_2p20 (64 x 2^20), c2c_2p16 (1024 x 2^16), c2c_2p12, r2c_2p22
 (16 x 2^22), r2c_2p20 (64 x 2^20; r2c_2p20_tr with the complex four-step and
 its real pass), c2r_2p20 (64 x 2^20), c2r_2p22 (16 x 2^22), c2c_2p25 (2 x 2^25,
-three passes), c2c_256c / r2c_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
+three passes), c2c_256c / r2c_256c / c2r_256c (256^3), r2c_8192s (8192^2), c2c_2p16_4096 (4096 x 2^16)
 and c2c_2p16_4096_mfma (the same in the split-precision MFMA mode), the short
 rows c2c_2p4 (2^21 x 16) and r2c_2p5 (2^20 x 32), the longest one-pass rows
 c2c_2p15 (1024 x 2^15) and c2r_2p16 (512 x 2^16), the three-pass c2c_2p23
@@ -72,6 +72,7 @@ def main():
             "r2c_2p20_tr": ("r", (1 << 20,), 64),
             "c2c_256c": ("c", (256, 256, 256), 1),
             "r2c_256c": ("r", (256, 256, 256), 1),
+            "c2r_256c": ("ri", (256, 256, 256), 1),
             "r2c_8192s": ("r", (8192, 8192), 1),
             "c2r_8192s": ("ri", (8192, 8192), 1),
             "c2r_2p20": ("ri", (1 << 20,), 64),

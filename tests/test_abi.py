@@ -135,7 +135,9 @@ def model_check(kind, dims, fmt, batch, prec=32, variant=None):
 SHAPES = [(1,), (2,), (4,), (8,), (32,), (64,), (4096,), (16384,), (32768,), (1 << 17,), (16, 32), (8, 8, 8),
           (2, 4), (4, 1, 8), (8192, 4), (2, 2, 2, 2), (32, 16384), (4, 32768),
           # long strided axes / long real dim 0: column four-step (+ real pass)
-          (8192, 8), (16384, 2), (4096, 2, 2)]
+          (8192, 8), (16384, 2), (4096, 2, 2),
+          # short real dim 0 (64- / 128-point real column passes on 64-column tiles)
+          (256, 8), (128, 16)]
 
 
 @pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])

@@ -194,6 +194,58 @@ def test_nd_c2r_real_pass_forms(G, shape):
     assert any("real_rows" in d for s_ in seen for d in s_), seen
 
 
+@pytest.mark.parametrize("shape", [(256, 8), (128, 16), (256, 64, 4), (128, 512), (256, 256, 2)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_nd_real_short_dim0_column_tiles(G, shape):
+    """N-D r2c / c2r with a dim 0 of 128 or 256 reals: its 64- or 128-point
+    column pass with the real post-pass (r2c) or pre-pass (c2r) on the default
+    tile (64 columns where compiled) and on forced 16- and 64-column tiles, in
+    both formats and a batch of 2 (shapes with fewer columns than a tile
+    included), each against the oracle."""
+    import torch
+    from fft_amd.afft import Plan
+    n = int(np.prod(shape))
+    hs = G.half(shape)
+    m = int(np.prod(hs))
+    xr = O.normal(13, 2 * n).reshape((2,) + shape)
+    refr = [O.rfft(xr[b].astype(np.float64)).ravel() for b in range(2)]
+    z = np.stack([O.hermitian_input(17 + b, shape) for b in range(2)])
+    refi = [O.irfft(z[b].astype(np.complex128), shape).ravel() for b in range(2)]
+    zf = z.reshape(2, m)
+    src_r = G.to_dev(xr.ravel())
+    for fmt in (G.IL, G.SP):
+        pr = Plan("r", shape, 2, format=fmt)
+        pi = Plan("ri", shape, 2, format=fmt)
+        for ct in (0, 16, 64):
+            pr.set_variant(col_tile=ct)
+            pi.set_variant(col_tile=ct)
+            say("nd real short dim0", shape, fmt, ct, pr.describe()[0], pi.describe()[-1])
+            if fmt == G.IL:
+                dst = G.empty_dev(4 * m)
+                pr(src_r, None, dst, None)
+                torch.cuda.synchronize()
+                y = G.to_host(dst)
+                y = (y[0::2] + 1j * y[1::2]).reshape(2, m)
+            else:
+                dre, dim_ = G.empty_dev(2 * m), G.empty_dev(2 * m)
+                pr(src_r, None, dre, dim_)
+                torch.cuda.synchronize()
+                y = (G.to_host(dre) + 1j * G.to_host(dim_)).reshape(2, m)
+            dst = G.empty_dev(2 * n)
+            if fmt == G.IL:
+                pi(G.to_dev(np.stack([zf.real, zf.imag], -1).astype(np.float32).ravel()), None, dst, None)
+            else:
+                pi(G.to_dev(zf.real.astype(np.float32).ravel()), G.to_dev(zf.imag.astype(np.float32).ravel()), dst,
+                   None)
+            torch.cuda.synchronize()
+            w = G.to_host(dst).reshape(2, n)
+            for b in range(2):
+                assert O.rel_l2(y[b].astype(np.complex64), refr[b]) <= O.bound(n), (shape, fmt, ct, b)
+                assert O.rel_l2(w[b].astype(np.float32), refi[b]) <= O.bound(n), (shape, fmt, ct, b)
+        pr.close()
+        pi.close()
+
+
 @pytest.mark.parametrize("shape", [(16,), (4096,), (1 << 14,), (1 << 15,), (16, 32), (8, 8, 8), (1 << 18,), (8192, 64),
                                    (16384, 64), (65536, 16), (2, 16384)],
                          ids=lambda s: "x".join(map(str, s)))
