@@ -430,6 +430,8 @@ struct Builder {
     // but c2r with 64-point columns 0.087 -> 0.092, so those keep the default
     // tile; col_tile = 64 still selects the wide one)
     if (pref_ct <= 0 && want <= 0 && colmap && L <= 128 && !(epi == EPI_C2R && L < 128)) pref_ct = 4 * deflt;
+    // and twice the default tile for the 256-point dim-0 r2c columns
+    if (pref_ct <= 0 && want <= 0 && colmap && epi == EPI_R2C && L == 256) pref_ct = 2 * deflt;
     struct Cand {
       int ct, hx_only;
     };

@@ -137,7 +137,7 @@ SHAPES = [(1,), (2,), (4,), (8,), (32,), (64,), (4096,), (16384,), (32768,), (1 
           # long strided axes / long real dim 0: column four-step (+ real pass)
           (8192, 8), (16384, 2), (4096, 2, 2),
           # short real dim 0 (64- / 128-point real column passes on 64-column tiles)
-          (256, 8), (128, 16)]
+          (256, 8), (128, 16), (512, 8), (1024, 8)]
 
 
 @pytest.mark.parametrize("kind", ["c", "ci", "r", "ri"])

@@ -194,10 +194,11 @@ def test_nd_c2r_real_pass_forms(G, shape):
     assert any("real_rows" in d for s_ in seen for d in s_), seen
 
 
-@pytest.mark.parametrize("shape", [(256, 8), (128, 16), (256, 64, 4), (128, 512), (256, 256, 2)],
+@pytest.mark.parametrize("shape", [(256, 8), (128, 16), (256, 64, 4), (128, 512), (256, 256, 2), (512, 8),
+                                   (1024, 64), (512, 32, 4)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_nd_real_short_dim0_column_tiles(G, shape):
-    """N-D r2c / c2r with a dim 0 of 128 or 256 reals: its 64- or 128-point
+    """N-D r2c / c2r with a dim 0 of 128 to 1024 reals: its 64- to 512-point
     column pass with the real post-pass (r2c) or pre-pass (c2r) on the default
     tile (64 columns where compiled) and on forced 16- and 64-column tiles, in
     both formats and a batch of 2 (shapes with fewer columns than a tile
